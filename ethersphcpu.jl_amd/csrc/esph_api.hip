@@ -1,0 +1,804 @@
+// esph_api.hip — host runtime behind the C ABI declared in include/ethersph.h.
+//
+// A scene owns its device buffers (two sorted copies of the fluid state, step
+// temporaries, sort scratch, the static wall set) plus a SceneDev table entry
+// describing them.  Stepping launches the kernels of esph_kernels.hip on the
+// context's stream; nothing is synchronised unless the caller asks for data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/ethersph.h"
+#include "esph_device.hpp"
+#include "esph_launch.hpp"
+#include "esph_runtime.hpp"
+
+using namespace esph;
+
+namespace esph {
+
+int fail(esph_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(esph_ctx* ctx, const char* what, hipError_t e) {
+    return fail(ctx, ESPH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* Arena::get(size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    total += bytes;
+    return p;
+}
+
+void Arena::release() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+    total = 0;
+}
+
+// SmoothKernel(R, dim, K) derived quantities, SPHKernel/SmoothKernel/SmoothKernel.jl:37-48.
+bool kernel_params(int kind, int dim, double R, double* h, double* sigma) {
+    static const double PI = 3.14159265358979323846;
+    double ratio, c[3];
+    switch (kind) {
+        case ESPH_CUBIC_SPLINE: ratio = 2.0; c[0] = 2.0 / 3.0; c[1] = 10.0 / 7.0 / PI; c[2] = 1.0 / PI; break;
+        case ESPH_GAUSSIAN: ratio = 3.0; c[0] = 1.0 / std::sqrt(PI); c[1] = 1.0 / PI; c[2] = 1.0 / std::sqrt(PI * PI * PI); break;
+        case ESPH_WENDLAND_C2: ratio = 2.0; c[0] = 0.0; c[1] = 7.0 / 4.0 / PI; c[2] = 21.0 / 16.0 / PI; break;
+        case ESPH_WENDLAND_C4: ratio = 2.0; c[0] = 5.0 / 8.0; c[1] = 9.0 / 4.0 / PI; c[2] = 495.0 / 256.0 / PI; break;
+        default: return false;
+    }
+    if (dim < 1 || dim > 3) return false;
+    *h = R / ratio;
+    double hd = *h;
+    if (dim == 2) hd = *h * *h;
+    if (dim == 3) hd = *h * *h * *h;
+    *sigma = c[dim - 1] / hd;
+    return true;
+}
+
+}  // namespace esph
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int esph_abi_version(void) { return ESPH_ABI_VERSION; }
+
+extern "C" int esph_ctx_create(int32_t device, uint64_t hbm_budget_bytes, esph_ctx** out) {
+    if (!out) return ESPH_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ESPH_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return ESPH_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return ESPH_ERR_HIP;
+    esph_ctx* c = new esph_ctx();
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ESPH_ERR_HIP;
+    }
+    *out = c;
+    return ESPH_OK;
+}
+
+extern "C" int esph_ctx_destroy(esph_ctx* ctx) {
+    if (!ctx) return ESPH_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ESPH_OK;
+}
+
+extern "C" const char* esph_last_error(const esph_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int esph_ctx_memory_used(const esph_ctx* ctx, uint64_t* bytes) {
+    if (!ctx || !bytes) return ESPH_ERR_ARG;
+    *bytes = ctx->used;
+    return ESPH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernel evaluation on the device
+// ---------------------------------------------------------------------------
+extern "C" int esph_kernel_eval(esph_ctx* ctx, int32_t kernel, int32_t dim, double influence_radius, int64_t n,
+                                const double* r, double* w, double* dw) {
+    if (!ctx || n < 0 || (n > 0 && (!r || !w || !dw))) return fail(ctx, ESPH_ERR_ARG, "esph_kernel_eval: bad argument");
+    double h, sigma;
+    if (!kernel_params(kernel, dim, influence_radius, &h, &sigma) || !(influence_radius > 0))
+        return fail(ctx, ESPH_ERR_ARG, "esph_kernel_eval: bad kernel/dim/radius");
+    if (n == 0) return ESPH_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    Arena a;
+    double* dr = (double*)a.get(sizeof(double) * n);
+    double* dw0 = (double*)a.get(sizeof(double) * n);
+    double* dw1 = (double*)a.get(sizeof(double) * n);
+    if (!dr || !dw0 || !dw1) { a.release(); return fail(ctx, ESPH_ERR_HIP, "esph_kernel_eval: allocation failed"); }
+    hipError_t e = hipMemcpyAsync(dr, r, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_kernel_eval(kernel, n, dr, dw0, dw1, h, sigma, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(w, dw0, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dw, dw1, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    a.release();
+    if (e != hipSuccess) return hip_fail(ctx, "esph_kernel_eval", e);
+    return ESPH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// scenes
+// ---------------------------------------------------------------------------
+extern "C" int esph_scene_create(esph_ctx* ctx, const esph_scene_desc* desc, esph_scene** out) {
+    if (!ctx || !desc || !out) return fail(ctx, ESPH_ERR_ARG, "esph_scene_create: null argument");
+    *out = nullptr;
+    const esph_scene_desc& d = *desc;
+    if (d.dim != 2 && d.dim != 3) return fail(ctx, ESPH_ERR_ARG, "dim must be 2 or 3");
+    double h, sigma;
+    if (!(d.influence_radius > 0) || !kernel_params(d.kernel, d.dim, d.influence_radius, &h, &sigma))
+        return fail(ctx, ESPH_ERR_ARG, "bad kernel or influence_radius");
+    if (d.kernel == ESPH_WENDLAND_C2 && d.dim == 1) return fail(ctx, ESPH_ERR_ARG, "WendlandC2 has no 1D form");
+    if (d.solver < ESPH_SOLVER_FLUID_WALL || d.solver > ESPH_SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL)
+        return fail(ctx, ESPH_ERR_ARG, "bad solver");
+    if (d.precision != ESPH_FP64 && d.precision != ESPH_FP32_BULK) return fail(ctx, ESPH_ERR_ARG, "bad precision");
+    if (d.gamma < 1) return fail(ctx, ESPH_ERR_ARG, "gamma must be >= 1");
+    if (!(d.rho_0 > 0) || !(d.dt > 0)) return fail(ctx, ESPH_ERR_ARG, "rho_0 and dt must be positive");
+    esph_scene* s = new esph_scene();
+    s->ctx = ctx;
+    s->d = d;
+    s->kh = h;
+    s->ksigma = sigma;
+    *out = s;
+    return ESPH_OK;
+}
+
+static void free_scene_device(esph_scene* s) {
+    if (s->timing_events.size()) {
+        for (hipEvent_t e : s->timing_events) (void)hipEventDestroy(e);
+        s->timing_events.clear();
+    }
+    s->ctx->used -= std::min<uint64_t>(s->ctx->used, s->arena.total);
+    s->arena.release();
+    s->allocated = false;
+    s->fluid_ready = false;
+}
+
+extern "C" int esph_scene_destroy(esph_scene* s) {
+    if (!s) return ESPH_ERR_ARG;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    free_scene_device(s);
+    delete s;
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_set_walls(esph_scene* s, int32_t ngroups, const esph_wall_arrays* g) {
+    if (!s || ngroups < 0 || (ngroups > 0 && !g)) return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "set_walls: bad argument");
+    if (s->allocated) return fail(s->ctx, ESPH_ERR_STATE, "set_walls must precede the first set_fluid");
+    const int D = s->d.dim;
+    int64_t nw = 0;
+    for (int q = 0; q < ngroups; ++q) {
+        if (g[q].n < 0 || (g[q].n > 0 && (!g[q].x || !g[q].normal || !g[q].gap)))
+            return fail(s->ctx, ESPH_ERR_ARG, "set_walls: group missing x/normal/gap");
+        if (g[q].kind < ESPH_WALL_COMPULSIVE || g[q].kind > ESPH_WALL_THERMOSTATIC)
+            return fail(s->ctx, ESPH_ERR_ARG, "set_walls: bad wall kind");
+        if (g[q].kind == ESPH_WALL_STATIC_VELOCITY && g[q].n > 0 && !g[q].v)
+            return fail(s->ctx, ESPH_ERR_ARG, "set_walls: static-velocity walls need v");
+        if (g[q].kind == ESPH_WALL_THERMOSTATIC && g[q].n > 0 && (!g[q].t || !g[q].kappa))
+            return fail(s->ctx, ESPH_ERR_ARG, "set_walls: thermostatic walls need t and kappa");
+        nw += g[q].n;
+    }
+    if (nw > INT32_MAX / 4) return fail(s->ctx, ESPH_ERR_ARG, "set_walls: too many wall particles");
+    WallHost& w = s->walls;
+    w = WallHost();
+    w.n = nw;
+    w.x.resize(nw * D); w.n_.resize(nw * D); w.v.assign(nw * D, 0.0);
+    w.gap.resize(nw); w.T.assign(nw, 0.0); w.kappa.assign(nw, 0.0); w.kind.resize(nw);
+    int64_t o = 0;
+    for (int q = 0; q < ngroups; ++q) {
+        const int64_t n = g[q].n;
+        if (n == 0) continue;
+        std::memcpy(&w.x[o * D], g[q].x, sizeof(double) * n * D);
+        std::memcpy(&w.n_[o * D], g[q].normal, sizeof(double) * n * D);
+        std::memcpy(&w.gap[o], g[q].gap, sizeof(double) * n);
+        if (g[q].kind == ESPH_WALL_STATIC_VELOCITY) std::memcpy(&w.v[o * D], g[q].v, sizeof(double) * n * D);
+        if (g[q].kind == ESPH_WALL_THERMOSTATIC) {
+            std::memcpy(&w.T[o], g[q].t, sizeof(double) * n);
+            std::memcpy(&w.kappa[o], g[q].kappa, sizeof(double) * n);
+        }
+        for (int64_t i = 0; i < n; ++i) w.kind[o + i] = g[q].kind;
+        o += n;
+    }
+    s->walls_set = true;
+    return ESPH_OK;
+}
+
+// Cell grid: cells 1.0001 R wide over the bounding box of fluid, walls and
+// finite check bounds (or the caller's domain), one cell of padding.
+static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) {
+    const int D = s->d.dim;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    auto add = [&](const double* x, int64_t n) {
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < D; ++k) {
+                const double v = x[i * D + k];
+                if (std::isfinite(v)) { lo[k] = std::min(lo[k], v); hi[k] = std::max(hi[k], v); }
+            }
+    };
+    if (s->d.domain_set) {
+        for (int k = 0; k < D; ++k) { lo[k] = s->d.domain_min[k]; hi[k] = s->d.domain_max[k]; }
+    } else {
+        add(f->x, f->n);
+        if (s->walls.n) add(s->walls.x.data(), s->walls.n);
+        if (s->d.check_enabled)
+            for (int k = 0; k < D; ++k) {
+                if (std::isfinite(s->d.check_min[k])) lo[k] = std::min(lo[k], s->d.check_min[k]);
+                if (std::isfinite(s->d.check_max[k])) hi[k] = std::max(hi[k], s->d.check_max[k]);
+            }
+    }
+    for (int k = 0; k < D; ++k)
+        if (!(lo[k] <= hi[k])) { lo[k] = 0.0; hi[k] = 0.0; }
+    double cs = s->d.influence_radius * 1.0001;
+    for (;;) {
+        int64_t ncell = 1;
+        for (int k = 0; k < 3; ++k) {
+            if (k >= D) { hd.nc[k] = 1; hd.lo[k] = 0.0; continue; }
+            hd.lo[k] = lo[k] - cs;
+            const int64_t n = (int64_t)std::floor((hi[k] - hd.lo[k]) / cs) + 2;
+            hd.nc[k] = (int32_t)std::min<int64_t>(n, 1 << 20);
+            ncell *= hd.nc[k];
+        }
+        if (ncell <= (int64_t)1 << 27) { hd.ncell = (int32_t)ncell; break; }
+        cs *= 1.25;  // wider cells keep the search exact
+    }
+    hd.inv_cs = 1.0 / cs;
+}
+
+static int cell_of_host(const SceneDev& hd, int D, const double* x) {
+    auto cc = [&](int k) {
+        const double f = std::floor((x[k] - hd.lo[k]) * hd.inv_cs);
+        if (!(f >= 0.0)) return 0;
+        if (f >= (double)(hd.nc[k] - 1)) return hd.nc[k] - 1;
+        return (int)f;
+    };
+    int c = cc(0) + hd.nc[0] * cc(1);
+    if (D == 3) c += hd.nc[0] * hd.nc[1] * cc(2);
+    return c;
+}
+
+// Allocates every device buffer of the scene (capacity cap) and fills the
+// static parts of SceneDev.
+static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
+    esph_ctx* ctx = s->ctx;
+    const esph_scene_desc& d = s->d;
+    const int D = d.dim;
+    int64_t cap64 = d.fluid_capacity > 0 ? d.fluid_capacity : f->n;
+    if (cap64 < f->n) return fail(ctx, ESPH_ERR_ARG, "fluid_capacity smaller than the upload");
+    if (cap64 > INT32_MAX / 4) return fail(ctx, ESPH_ERR_ARG, "fluid capacity too large");
+    const int cap = (int)std::max<int64_t>(cap64, 1);
+    SceneDev hd{};
+    hd.dim = D;
+    hd.kernel = d.kernel;
+    hd.solver = d.solver;
+    hd.prec = d.precision;
+    const bool thermal = d.solver == ESPH_SOLVER_FLUID_THERMOSTATIC_WALL ||
+                         d.solver == ESPH_SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL;
+    hd.flags = (thermal ? FLAG_THERMAL : 0) | (d.solver != ESPH_SOLVER_FLUID_WALL ? FLAG_WALL_PRESSURE : 0) |
+               (d.check_enabled ? FLAG_CHECK : 0);
+    hd.gamma = d.gamma;
+    hd.cap = cap;
+    hd.kreinit = d.density_reinitialized_interval;
+    hd.kcheck = d.check_interval;
+    hd.R = d.influence_radius;
+    hd.R2 = d.influence_radius * d.influence_radius;
+    hd.h = s->kh;
+    hd.sigma = s->ksigma;
+    hd.W0 = s->ksigma;
+    hd.rho0 = d.rho_0;
+    hd.c0 = d.c_0;
+    hd.p0 = d.p_0;
+    hd.b = d.c_0 * d.c_0 * d.rho_0 / (double)d.gamma;  // LiquidModel.jl:30
+    hd.nu0 = d.mu_0 / d.rho_0;                          // LiquidModel.jl:31
+    hd.t0 = d.t_0;
+    hd.cp = d.cp;
+    hd.beta = d.beta;
+    hd.dt = d.dt;
+    for (int k = 0; k < 3; ++k) {
+        hd.g[k] = k < D ? d.body_force[k] : 0.0;
+        hd.check_min[k] = d.check_min[k];
+        hd.check_max[k] = d.check_max[k];
+    }
+    build_grid(s, f, hd);
+    const int nw = (int)s->walls.n;
+    hd.nw = nw;
+    hd.cell_tiles = (hd.ncell + 1 + kCellTile - 1) / kCellTile;
+
+    // size check against the context budget
+    const bool f32 = d.precision == ESPH_FP32_BULK;
+    const size_t F = sizeof(float), G = sizeof(double), I = sizeof(int32_t);
+    size_t per_set = (size_t)cap * (2 * D * G + 5 * G + I + (f32 ? F : G)) +
+                     (thermal ? (size_t)cap * 3 * G : 0) +
+                     (f32 ? (size_t)cap * (D * F + 3 * F + (thermal ? 3 * F : 0)) : 0);
+    size_t bytes = 2 * per_set + (size_t)cap * (2 * D * G + G + (thermal ? G : 0) + 3 * I) +
+                   (size_t)(hd.ncell + 1) * I * 2 + I + (size_t)hd.cell_tiles * I +
+                   (size_t)nw * (3 * D * G + 3 * G + 2 * I) + (size_t)(hd.ncell + 1) * I +
+                   (f32 ? (size_t)nw * (2 * D * F + 3 * F) : 0) + sizeof(SceneDyn) + sizeof(SceneDev) +
+                   ((size_t)(cap + kBlock - 1) / kBlock + (size_t)hd.cell_tiles) * sizeof(Tile);
+    if (ctx->budget && ctx->used + bytes > ctx->budget)
+        return fail(ctx, ESPH_ERR_BUDGET, "scene needs " + std::to_string(bytes) + " bytes; budget left " +
+                                              std::to_string(ctx->budget - ctx->used));
+    Arena& a = s->arena;
+    bool ok = true;
+    auto dal = [&](size_t n) { void* p = a.get(n); ok = ok && p; return p; };
+    for (int q = 0; q < 2; ++q) {
+        FluidSet& S = hd.set[q];
+        S.x = (double*)dal(G * cap * D);
+        S.v = (double*)dal(G * cap * D);
+        S.rho = (double*)dal(G * cap);
+        S.p = (double*)dal(G * cap);
+        S.c = (double*)dal(G * cap);
+        S.m = (double*)dal(G * cap);
+        S.gap = (double*)dal(G * cap);
+        S.id = (int32_t*)dal(I * cap);
+        S.pr2s = dal((f32 ? F : G) * cap);
+        if (thermal) {
+            S.T = (double*)dal(G * cap);
+            S.kappa = (double*)dal(G * cap);
+            S.cp = (double*)dal(G * cap);
+        }
+        if (f32) {
+            S.vs = dal(F * cap * D);
+            S.ms = dal(F * cap);
+            S.rhos = dal(F * cap);
+            S.gaps = dal(F * cap);
+            if (thermal) { S.Ts = dal(F * cap); S.kappas = dal(F * cap); S.cps = dal(F * cap); }
+        } else {
+            S.vs = S.v; S.ms = S.m; S.rhos = S.rho; S.gaps = S.gap;
+            S.Ts = S.T; S.kappas = S.kappa; S.cps = S.cp;
+        }
+    }
+    hd.xn = (double*)dal(G * cap * D);
+    hd.vn = (double*)dal(G * cap * D);
+    hd.rhon = (double*)dal(G * cap);
+    hd.Tn = thermal ? (double*)dal(G * cap) : nullptr;
+    hd.key = (int32_t*)dal(I * cap);
+    hd.slot = (int32_t*)dal(I * cap);
+    hd.tmp = (int32_t*)dal(I * cap);
+    hd.count = (int32_t*)dal(I * (hd.ncell + 1));
+    hd.cell_start = (int32_t*)dal(I * (hd.ncell + 2));
+    hd.bsum = (int32_t*)dal(I * hd.cell_tiles);
+    // walls, binned on the host once (stable by upload index)
+    hd.wx = (double*)dal(G * std::max(nw, 1) * D);
+    hd.wn = (double*)dal(G * std::max(nw, 1) * D);
+    hd.wv = (double*)dal(G * std::max(nw, 1) * D);
+    hd.wgap = (double*)dal(G * std::max(nw, 1));
+    hd.wT = (double*)dal(G * std::max(nw, 1));
+    hd.wkappa = (double*)dal(G * std::max(nw, 1));
+    hd.wkind = (int32_t*)dal(I * std::max(nw, 1));
+    hd.wid = (int32_t*)dal(I * std::max(nw, 1));
+    hd.wcell_start = (int32_t*)dal(I * (hd.ncell + 1));
+    if (f32) {
+        hd.wns = dal(F * std::max(nw, 1) * D);
+        hd.wvs = dal(F * std::max(nw, 1) * D);
+        hd.wgaps = dal(F * std::max(nw, 1));
+        hd.wTs = dal(F * std::max(nw, 1));
+        hd.wkappas = dal(F * std::max(nw, 1));
+    } else {
+        hd.wns = hd.wn; hd.wvs = hd.wv; hd.wgaps = hd.wgap; hd.wTs = hd.wT; hd.wkappas = hd.wkappa;
+    }
+    hd.dyn = (SceneDyn*)dal(sizeof(SceneDyn));
+    s->dev = (SceneDev*)dal(sizeof(SceneDev));
+    s->nptiles = (cap + kBlock - 1) / kBlock;
+    s->nctiles = hd.cell_tiles;
+    s->ptiles = (Tile*)dal(sizeof(Tile) * s->nptiles);
+    s->ctiles = (Tile*)dal(sizeof(Tile) * s->nctiles);
+    if (!ok) {
+        a.release();
+        return fail(ctx, ESPH_ERR_HIP, "scene allocation failed (hipMalloc)");
+    }
+    ctx->used += a.total;
+    hipStream_t st = ctx->stream;
+    // wall binning and upload
+    {
+        const WallHost& w = s->walls;
+        std::vector<int> key(nw), order(nw);
+        for (int i = 0; i < nw; ++i) key[i] = cell_of_host(hd, D, &w.x[(size_t)i * D]);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a1, int b1) { return key[a1] < key[b1]; });
+        std::vector<double> x(nw * D), nn(nw * D), v(nw * D), gap(nw), T(nw), kap(nw);
+        std::vector<int32_t> kind(nw), wid(nw), wcs(hd.ncell + 1, 0);
+        for (int p = 0; p < nw; ++p) {
+            const int i = order[p];
+            for (int k = 0; k < D; ++k) {
+                x[p * D + k] = w.x[i * D + k];
+                nn[p * D + k] = w.n_[i * D + k];
+                v[p * D + k] = w.v[i * D + k];
+            }
+            gap[p] = w.gap[i]; T[p] = w.T[i]; kap[p] = w.kappa[i]; kind[p] = w.kind[i]; wid[p] = i;
+        }
+        for (int i = 0; i < nw; ++i) wcs[key[i] + 1]++;
+        for (int c = 0; c < hd.ncell; ++c) wcs[c + 1] += wcs[c];
+        s->wall_stage = WallStage{x, nn, v, gap, T, kap, kind, wid, wcs};
+        WallStage& ws = s->wall_stage;
+        hipError_t e = hipSuccess;
+        auto up = [&](void* dst, const void* src, size_t n) {
+            if (e == hipSuccess && n) e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st);
+        };
+        up(hd.wx, ws.x.data(), G * nw * D);
+        up(hd.wn, ws.n_.data(), G * nw * D);
+        up(hd.wv, ws.v.data(), G * nw * D);
+        up(hd.wgap, ws.gap.data(), G * nw);
+        up(hd.wT, ws.T.data(), G * nw);
+        up(hd.wkappa, ws.kappa.data(), G * nw);
+        up(hd.wkind, ws.kind.data(), I * nw);
+        up(hd.wid, ws.wid.data(), I * nw);
+        up(hd.wcell_start, ws.wcs.data(), I * (hd.ncell + 1));
+        if (f32) {
+            auto tof = [](const std::vector<double>& src) {
+                std::vector<float> o(src.size());
+                for (size_t i = 0; i < src.size(); ++i) o[i] = (float)src[i];
+                return o;
+            };
+            ws.fn = tof(ws.n_); ws.fv = tof(ws.v); ws.fgap = tof(ws.gap); ws.fT = tof(ws.T); ws.fkap = tof(ws.kappa);
+            up(hd.wns, ws.fn.data(), F * nw * D);
+            up(hd.wvs, ws.fv.data(), F * nw * D);
+            up(hd.wgaps, ws.fgap.data(), F * nw);
+            up(hd.wTs, ws.fT.data(), F * nw);
+            up(hd.wkappas, ws.fkap.data(), F * nw);
+        }
+        std::vector<Tile> pt(s->nptiles), ct(s->nctiles);
+        for (int t = 0; t < s->nptiles; ++t) pt[t] = Tile{0, t * kBlock};
+        for (int t = 0; t < s->nctiles; ++t) ct[t] = Tile{0, t * kCellTile};
+        s->tile_stage_p = pt;
+        s->tile_stage_c = ct;
+        up(s->ptiles, s->tile_stage_p.data(), sizeof(Tile) * pt.size());
+        up(s->ctiles, s->tile_stage_c.data(), sizeof(Tile) * ct.size());
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(ctx, "wall upload", e);
+    }
+    s->h = hd;
+    s->allocated = true;
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
+    if (!s || !f) return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "set_fluid: null argument");
+    esph_ctx* ctx = s->ctx;
+    const int D = s->d.dim;
+    const bool thermal = s->d.solver == ESPH_SOLVER_FLUID_THERMOSTATIC_WALL ||
+                         s->d.solver == ESPH_SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL;
+    if (f->n < 0 || (f->n > 0 && (!f->x || !f->v || !f->rho || !f->p || !f->c || !f->mass || !f->gap)))
+        return fail(ctx, ESPH_ERR_ARG, "set_fluid: missing arrays");
+    if (thermal && f->n > 0 && (!f->t || !f->kappa || !f->cp))
+        return fail(ctx, ESPH_ERR_ARG, "set_fluid: thermal solvers need t, kappa and cp");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (!s->allocated) {
+        int rc = allocate_scene(s, f);
+        if (rc) return rc;
+        HIPCHK(ctx, hipMemcpy(s->dev, &s->h, sizeof(SceneDev), hipMemcpyHostToDevice));
+    }
+    SceneDev& hd = s->h;
+    if (f->n > hd.cap) return fail(ctx, ESPH_ERR_CAPACITY, "set_fluid: more particles than the scene capacity");
+    const int n = (int)f->n;
+    hipStream_t st = ctx->stream;
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    const size_t G = sizeof(double);
+    FluidSet& S = hd.set[0];
+    std::vector<int32_t> ids(n);
+    std::iota(ids.begin(), ids.end(), 0);
+    hipError_t e = hipSuccess;
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    };
+    up(hd.xn, f->x, G * n * D);
+    up(hd.vn, f->v, G * n * D);
+    up(hd.rhon, f->rho, G * n);
+    up(S.p, f->p, G * n);
+    up(S.c, f->c, G * n);
+    up(S.m, f->mass, G * n);
+    up(S.gap, f->gap, G * n);
+    up(S.id, ids.data(), sizeof(int32_t) * n);
+    if (thermal) {
+        up(hd.Tn, f->t, G * n);
+        up(S.kappa, f->kappa, G * n);
+        up(S.cp, f->cp, G * n);
+    }
+    SceneDyn dyn{};
+    dyn.step = 0;
+    dyn.parity = 0;
+    dyn.n_live = n;
+    up(hd.dyn, &dyn, sizeof(SceneDyn));
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, "set_fluid upload", e);
+    SortPlan plan{s->dev, 1, s->ptiles, s->nptiles, s->ctiles, s->nctiles};
+    e = launch_sort(plan, 1, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, "set_fluid sort", e);
+    s->step = 0;
+    s->n_upload = n;
+    s->fluid_ready = true;
+    return ESPH_OK;
+}
+
+static Bucket scene_bucket(esph_scene* s) {
+    return Bucket{s->d.dim, s->d.kernel, s->d.precision, s->dev, s->ptiles, s->nptiles};
+}
+
+static hipEvent_t timing_event(esph_scene* s) {
+    if (s->ev_next < s->timing_events.size()) return s->timing_events[s->ev_next++];
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    s->timing_events.push_back(e);
+    s->ev_next++;
+    return e;
+}
+
+extern "C" int esph_scene_step(esph_scene* s, int64_t nsteps) {
+    if (!s || nsteps < 0) return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "step: bad argument");
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "step before set_fluid");
+    esph_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const Bucket b = scene_bucket(s);
+    const SortPlan plan{s->dev, 1, s->ptiles, s->nptiles, s->ctiles, s->nctiles};
+    const int64_t K = s->d.density_reinitialized_interval;
+    for (int64_t t = 0; t < nsteps; ++t) {
+        const bool shep = K > 0 && (s->step + 1) % K == 0;
+        hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        if (s->timing) {
+            for (int q = 0; q < 5; ++q) ev[q] = timing_event(s);
+            (void)hipEventRecord(ev[0], st);
+        }
+        hipError_t e = launch_continuity(b, st);
+        if (s->timing) (void)hipEventRecord(ev[1], st);
+        if (e == hipSuccess) e = launch_forces(b, st);
+        if (s->timing) (void)hipEventRecord(ev[2], st);
+        if (e == hipSuccess && shep) e = launch_shepard(b, st);
+        if (s->timing) (void)hipEventRecord(ev[3], st);
+        if (e == hipSuccess) e = launch_sort(plan, 0, st);
+        if (s->timing) {
+            (void)hipEventRecord(ev[4], st);
+            s->timed.push_back(TimedStep{{ev[0], ev[1], ev[2], ev[3], ev[4]}, shep});
+        }
+        if (e != hipSuccess) return hip_fail(ctx, "step launch", e);
+        s->step += 1;
+    }
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_sync(esph_scene* s) {
+    if (!s) return ESPH_ERR_ARG;
+    esph_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!s->fluid_ready) return ESPH_OK;
+    SceneDyn dyn;
+    HIPCHK(ctx, hipMemcpy(&dyn, s->h.dyn, sizeof(SceneDyn), hipMemcpyDeviceToHost));
+    if (dyn.err & 1) return fail(ctx, ESPH_ERR_NAN, "a non-finite fluid position was met");
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_step_count(const esph_scene* s, int64_t* step) {
+    if (!s || !step) return ESPH_ERR_ARG;
+    *step = s->step;
+    return ESPH_OK;
+}
+
+static int read_dyn(esph_scene* s, SceneDyn* dyn) {
+    esph_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(dyn, s->h.dyn, sizeof(SceneDyn), hipMemcpyDeviceToHost));
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_fluid_count(esph_scene* s, int64_t* n) {
+    if (!s || !n) return ESPH_ERR_ARG;
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    SceneDyn dyn;
+    int rc = read_dyn(s, &dyn);
+    if (rc) return rc;
+    *n = dyn.n_live;
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_get_fluid(esph_scene* s, int32_t field, double* out, int64_t capacity, int64_t* written) {
+    if (!s || !out) return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "get_fluid: null argument");
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    esph_ctx* ctx = s->ctx;
+    const int D = s->d.dim;
+    const bool thermal = s->h.flags & FLAG_THERMAL;
+    SceneDyn dyn;
+    int rc = read_dyn(s, &dyn);
+    if (rc) return rc;
+    const int n = dyn.n_live;
+    const FluidSet& S = s->h.set[dyn.parity];
+    const double* src = nullptr;
+    int w = 1;
+    switch (field) {
+        case ESPH_FIELD_X: src = S.x; w = D; break;
+        case ESPH_FIELD_V: src = S.v; w = D; break;
+        case ESPH_FIELD_RHO: src = S.rho; break;
+        case ESPH_FIELD_P: src = S.p; break;
+        case ESPH_FIELD_C: src = S.c; break;
+        case ESPH_FIELD_MASS: src = S.m; break;
+        case ESPH_FIELD_GAP: src = S.gap; break;
+        case ESPH_FIELD_T: src = thermal ? S.T : nullptr; break;
+        case ESPH_FIELD_KAPPA: src = thermal ? S.kappa : nullptr; break;
+        case ESPH_FIELD_CP: src = thermal ? S.cp : nullptr; break;
+        case ESPH_FIELD_ID: break;
+        default: return fail(ctx, ESPH_ERR_ARG, "get_fluid: bad field");
+    }
+    if (field != ESPH_FIELD_ID && !src) return fail(ctx, ESPH_ERR_ARG, "get_fluid: field not present for this solver");
+    if (written) *written = (int64_t)n * w;
+    if (capacity < (int64_t)n * w) return fail(ctx, ESPH_ERR_CAPACITY, "get_fluid: output too small");
+    std::vector<int32_t> ids(n);
+    HIPCHK(ctx, hipMemcpy(ids.data(), S.id, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    // reference order = ascending upload index among live particles
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return ids[a] < ids[b]; });
+    if (field == ESPH_FIELD_ID) {
+        for (int q = 0; q < n; ++q) out[q] = (double)ids[order[q]];
+        return ESPH_OK;
+    }
+    std::vector<double> buf((size_t)n * w);
+    HIPCHK(ctx, hipMemcpy(buf.data(), src, sizeof(double) * n * w, hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; ++q)
+        for (int k = 0; k < w; ++k) out[(size_t)q * w + k] = buf[(size_t)order[q] * w + k];
+    return ESPH_OK;
+}
+
+// Pair export through k_pairs: count, host scan, write, remap, sort.
+static int export_pairs(esph_scene* s, int pairset, std::vector<int64_t>& pairs_out) {
+    esph_ctx* ctx = s->ctx;
+    SceneDyn dyn;
+    int rc = read_dyn(s, &dyn);
+    if (rc) return rc;
+    const int n = dyn.n_live;
+    hipStream_t st = ctx->stream;
+    Arena a;
+    int32_t* counts = (int32_t*)a.get(sizeof(int32_t) * std::max(n, 1));
+    int64_t* offs = (int64_t*)a.get(sizeof(int64_t) * std::max(n, 1));
+    if (!counts || !offs) { a.release(); return fail(ctx, ESPH_ERR_HIP, "export: allocation failed"); }
+    const Bucket b = scene_bucket(s);
+    hipError_t e = launch_pairs(b, pairset, counts, nullptr, nullptr, st);
+    std::vector<int32_t> hc(n);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(hc.data(), counts, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { a.release(); return hip_fail(ctx, "export count", e); }
+    std::vector<int64_t> ho(n);
+    int64_t m = 0;
+    for (int i = 0; i < n; ++i) { ho[i] = m; m += hc[i]; }
+    int32_t* dout = (int32_t*)a.get(sizeof(int32_t) * 2 * std::max<int64_t>(m, 1));
+    if (!dout) { a.release(); return fail(ctx, ESPH_ERR_HIP, "export: allocation failed"); }
+    std::vector<int32_t> hp(2 * m);
+    if (n) e = hipMemcpyAsync(offs, ho.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_pairs(b, pairset, nullptr, offs, dout, st);
+    if (e == hipSuccess && m) e = hipMemcpyAsync(hp.data(), dout, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost, st);
+    std::vector<int32_t> ids(n);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(ids.data(), s->h.set[dyn.parity].id, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    a.release();
+    if (e != hipSuccess) return hip_fail(ctx, "export write", e);
+    // upload index -> reference index (rank among live upload indices)
+    std::vector<int32_t> sorted_ids(ids);
+    std::sort(sorted_ids.begin(), sorted_ids.end());
+    std::vector<int32_t> rank(s->n_upload > 0 ? s->n_upload : 1, -1);
+    for (int q = 0; q < n; ++q) rank[sorted_ids[q]] = q;
+    pairs_out.resize(m);
+    for (int64_t q = 0; q < m; ++q) {
+        const int64_t i = rank[hp[2 * q]];
+        const int64_t j = pairset == ESPH_PAIRS_FLUID_FLUID ? rank[hp[2 * q + 1]] : hp[2 * q + 1];
+        pairs_out[q] = (i << 32) | j;
+    }
+    std::sort(pairs_out.begin(), pairs_out.end());
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_neighbours(esph_scene* s, int32_t pairset, int32_t* out_pairs, int64_t capacity, int64_t* m) {
+    if (!s || !m || (pairset != ESPH_PAIRS_FLUID_FLUID && pairset != ESPH_PAIRS_FLUID_WALL))
+        return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "neighbours: bad argument");
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    std::vector<int64_t> pairs;
+    int rc = export_pairs(s, pairset, pairs);
+    if (rc) return rc;
+    *m = (int64_t)pairs.size();
+    if (!out_pairs && capacity == 0) return ESPH_OK;
+    if ((int64_t)pairs.size() > capacity) return fail(s->ctx, ESPH_ERR_CAPACITY, "neighbours: output too small");
+    for (size_t q = 0; q < pairs.size(); ++q) {
+        out_pairs[2 * q] = (int32_t)(pairs[q] >> 32);
+        out_pairs[2 * q + 1] = (int32_t)(pairs[q] & 0xffffffff);
+    }
+    return ESPH_OK;
+}
+
+extern "C" int esph_neighbor_list(esph_ctx* ctx, int32_t dim, double cutoff, int64_t nx, const double* x, int64_t ny,
+                                  const double* y, int32_t* out_pairs, int64_t capacity, int64_t* m) {
+    if (!ctx || !m || (dim != 2 && dim != 3) || !(cutoff > 0) || nx < 0 || ny < 0 || (nx > 0 && !x) ||
+        (ny > 0 && !y))
+        return fail(ctx, ESPH_ERR_ARG, "neighbor_list: bad argument");
+    esph_scene_desc d{};
+    d.dim = dim;
+    d.kernel = ESPH_CUBIC_SPLINE;
+    d.influence_radius = cutoff;
+    d.solver = ESPH_SOLVER_FLUID_WALL;
+    d.precision = ESPH_FP64;
+    d.rho_0 = 1.0;
+    d.c_0 = 1.0;
+    d.gamma = 1;
+    d.dt = 1.0;
+    esph_scene* s = nullptr;
+    int rc = esph_scene_create(ctx, &d, &s);
+    if (rc) return rc;
+    std::vector<double> ones(std::max<int64_t>(std::max(nx, ny), 1), 1.0);
+    std::vector<double> zeros(std::max<int64_t>(std::max(nx, ny), 1) * dim, 0.0);
+    if (ny > 0) {
+        esph_wall_arrays w{};
+        w.kind = ESPH_WALL_COMPULSIVE;
+        w.n = ny;
+        w.x = y;
+        w.normal = zeros.data();
+        w.gap = ones.data();
+        rc = esph_scene_set_walls(s, 1, &w);
+    }
+    if (!rc) {
+        esph_fluid_arrays f{};
+        f.n = nx;
+        f.x = x;
+        f.v = zeros.data();
+        f.rho = ones.data();
+        f.p = zeros.data();
+        f.c = ones.data();
+        f.mass = ones.data();
+        f.gap = ones.data();
+        rc = esph_scene_set_fluid(s, &f);
+    }
+    if (!rc) rc = esph_scene_neighbours(s, ny > 0 ? ESPH_PAIRS_FLUID_WALL : ESPH_PAIRS_FLUID_FLUID, out_pairs,
+                                        capacity, m);
+    esph_scene_destroy(s);
+    return rc;
+}
+
+extern "C" int esph_scene_enable_timing(esph_scene* s, int32_t enable) {
+    if (!s) return ESPH_ERR_ARG;
+    s->timing = enable != 0;
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_phase_times(esph_scene* s, double* ms5, int64_t* steps) {
+    if (!s || !ms5) return ESPH_ERR_ARG;
+    esph_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    double acc[5] = {0, 0, 0, 0, 0};
+    int64_t nshep = 0;
+    for (const TimedStep& t : s->timed) {
+        float a;
+        (void)hipEventElapsedTime(&a, t.ev[0], t.ev[1]); acc[1] += a;
+        (void)hipEventElapsedTime(&a, t.ev[1], t.ev[2]); acc[2] += a;
+        (void)hipEventElapsedTime(&a, t.ev[2], t.ev[3]); if (t.shepard) { acc[3] += a; nshep++; }
+        (void)hipEventElapsedTime(&a, t.ev[3], t.ev[4]); acc[0] += a;
+    }
+    const int64_t n = (int64_t)s->timed.size();
+    for (int q = 0; q < 5; ++q) ms5[q] = 0.0;
+    if (n) {
+        ms5[0] = acc[0] / n;
+        ms5[1] = acc[1] / n;
+        ms5[2] = acc[2] / n;
+        ms5[3] = nshep ? acc[3] / nshep : 0.0;
+    }
+    if (steps) *steps = n;
+    s->timed.clear();
+    s->ev_next = 0;
+    return ESPH_OK;
+}

@@ -1,0 +1,256 @@
+// esph_device.hpp — device-side data structures and per-pair math of the
+// gfx950 SPH hot path.  Shared by the kernels (esph_kernels.hip) and the
+// host runtime (esph_api.hip), which fills SceneDev and uploads it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ethersph.h"
+
+namespace esph {
+
+constexpr int kBlock = 256;          // particles per tile == threads per block
+constexpr int kScanItems = 4;        // cells per thread in the scan kernels
+constexpr int kCellTile = kBlock * kScanItems;
+
+// SceneDev::flags
+enum : int {
+    FLAG_THERMAL = 1,        // thermal fluid + ThermalWeaklyCompressibleLiquidModel
+    FLAG_WALL_PRESSURE = 2,  // pressureForce!(fluid, wall) in the wall loop
+    FLAG_CHECK = 4,          // AABB deletion predicate enabled
+};
+
+// Per-scene state that changes every step and lives in device memory, so a
+// captured step graph can be replayed without the host.
+struct SceneDyn {
+    int64_t step;    // completed steps (the reference's `step` of the last one)
+    int32_t parity;  // which FluidSet is current
+    int32_t n_live;  // fluid particles in the current set
+    int32_t err;     // bit 0: a non-finite position was met
+    int32_t pad;
+};
+
+// One sorted copy of the fluid state.  Masters are binary64; the *s pointers
+// are what the pair loops read: binary32 shadows in FP32_BULK, the masters
+// themselves in FP64.
+struct FluidSet {
+    double* x;      // n*dim
+    double* v;      // n*dim
+    double* rho;
+    double* p;
+    double* c;
+    double* m;
+    double* gap;
+    double* T;
+    double* kappa;
+    double* cp;
+    int32_t* id;    // upload index
+    void* vs;       // n*dim
+    void* ms;
+    void* rhos;
+    void* pr2s;     // p / rho^2 (pressure term precomputed per particle)
+    void* gaps;
+    void* Ts;
+    void* kappas;
+    void* cps;
+};
+
+struct SceneDev {
+    int32_t dim, kernel, solver, prec;
+    int32_t flags;
+    int32_t gamma;
+    int32_t cap;         // fluid capacity
+    int32_t ncell;       // cells of the grid (the tail bucket is index ncell)
+    int32_t nc[3];
+    int32_t nw;          // wall particles
+    int64_t kreinit;     // density_reinitialized_interval (<=0: never)
+    int64_t kcheck;      // check_interval (<=0: never)
+    // kernel (SmoothKernel.jl:37-48)
+    double R, R2, h, sigma, W0;
+    // model (LiquidModel.jl:10-66)
+    double rho0, c0, p0, b, nu0, t0, cp, beta, dt;
+    double g[3];
+    double check_min[3], check_max[3];
+    // cell grid
+    double lo[3];
+    double inv_cs;
+    // fluid
+    FluidSet set[2];
+    double* xn;      // step outputs in the current set's order
+    double* vn;
+    double* Tn;
+    double* rhon;    // Shepard output / upload staging
+    int32_t* key;
+    int32_t* slot;
+    int32_t* count;       // ncell + 1
+    int32_t* cell_start;  // ncell + 2
+    int32_t* tmp;         // cap
+    int32_t* bsum;        // scan block sums
+    int32_t cell_tiles;   // ceil((ncell+1) / kCellTile)
+    int32_t pad0;
+    // walls: static, sorted by cell once
+    double* wx;      // nw*dim
+    double* wn;      // nw*dim
+    double* wgap;
+    double* wv;      // nw*dim (zeros for fixed walls)
+    double* wT;
+    double* wkappa;
+    int32_t* wkind;
+    int32_t* wid;    // wall index in group-concatenated upload order
+    int32_t* wcell_start;  // ncell + 1
+    void* wgaps;     // pair-precision views
+    void* wns;
+    void* wvs;
+    void* wTs;
+    void* wkappas;
+    SceneDyn* dyn;
+};
+
+struct Tile {
+    int32_t scene;   // index into the SceneDev table
+    int32_t begin;   // first particle (or cell) of the tile
+};
+
+// ---------------------------------------------------------------------------
+// Cell grid.  Cells are a little wider than R (host sets inv_cs), so two
+// particles within R are never more than one cell apart along any axis.
+// Positions outside the grid are clamped into the border cells, which keeps
+// the search exact.  Non-finite coordinates go to cell 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int cell_coord(double x, double lo, double inv, int nc) {
+    double f = floor((x - lo) * inv);
+    if (!(f >= 0.0)) return 0;
+    if (f >= (double)(nc - 1)) return nc - 1;
+    return (int)f;
+}
+
+template <int D>
+__device__ __forceinline__ void cell_of(const SceneDev& s, const double* x, int c[3]) {
+    c[0] = cell_coord(x[0], s.lo[0], s.inv_cs, s.nc[0]);
+    c[1] = cell_coord(x[1], s.lo[1], s.inv_cs, s.nc[1]);
+    c[2] = D == 3 ? cell_coord(x[2], s.lo[2], s.inv_cs, s.nc[2]) : 0;
+}
+
+__device__ __forceinline__ int cell_linear(const SceneDev& s, int cx, int cy, int cz) {
+    return cx + s.nc[0] * (cy + s.nc[1] * cz);
+}
+
+// Neighbour rule of the ABI: d2 in binary64, each operation rounded on its
+// own (no contraction), d2 <= R*R.
+template <int D>
+__device__ __forceinline__ double dist2_exact(const double* a, const double* b, double* rv) {
+#pragma clang fp contract(off)
+    double d0 = a[0] - b[0];
+    double d1 = a[1] - b[1];
+    double s = d0 * d0;
+    s = s + d1 * d1;
+    rv[0] = d0;
+    rv[1] = d1;
+    if (D == 3) {
+        double d2 = a[2] - b[2];
+        s = s + d2 * d2;
+        rv[2] = d2;
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Smoothing kernels, CubicSpline.jl / Gaussian.jl / WendlandC2.jl /
+// WendlandC4.jl.  Operation order follows the reference's expressions; the
+// binary64 form does not contract, so values equal a plain C evaluation.
+// ---------------------------------------------------------------------------
+template <typename T, int K>
+struct SmoothKernelFn;
+
+template <typename T>
+struct SmoothKernelFn<T, ESPH_CUBIC_SPLINE> {
+    static constexpr double ratio = 2.0;
+    __device__ static __forceinline__ T value(T r, T h, T sigma) {
+#pragma clang fp contract(off)
+        T q = r / h;
+        if (q < T(1)) return sigma / T(4) * (T(3) * (q * q) * (q - T(2)) + T(4));
+        if (q < T(2)) { T t = T(2) - q; return sigma / T(4) * (t * t * t); }
+        return T(0);
+    }
+    __device__ static __forceinline__ T gradient(T r, T h, T sigma) {
+#pragma clang fp contract(off)
+        T q = r / h;
+        if (q < T(1)) return sigma / h * T(3) / T(4) * q * (T(3) * q - T(4));
+        if (q < T(2)) { T t = T(2) - q; return -sigma / h * T(3) / T(4) * (t * t); }
+        return T(0);
+    }
+};
+
+template <typename T>
+struct SmoothKernelFn<T, ESPH_GAUSSIAN> {
+    static constexpr double ratio = 3.0;
+    __device__ static __forceinline__ T value(T r, T h, T sigma) {
+#pragma clang fp contract(off)
+        T q = r / h;
+        if (q < T(3)) return sigma * exp(-(q * q));
+        return T(0);
+    }
+    __device__ static __forceinline__ T gradient(T r, T h, T sigma) {
+#pragma clang fp contract(off)
+        T q = r / h;
+        if (q < T(3)) return T(-2) * sigma / h * q * exp(-(q * q));
+        return T(0);
+    }
+};
+
+template <typename T>
+struct SmoothKernelFn<T, ESPH_WENDLAND_C2> {
+    static constexpr double ratio = 2.0;
+    __device__ static __forceinline__ T value(T r, T h, T sigma) {
+#pragma clang fp contract(off)
+        T q = r / h;
+        if (q < T(2)) {
+            T t = T(2) - q, t2 = t * t;
+            return sigma * (t2 * t2) * (T(1) + T(2) * q) / T(16);
+        }
+        return T(0);
+    }
+    __device__ static __forceinline__ T gradient(T r, T h, T sigma) {
+#pragma clang fp contract(off)
+        T q = r / h;
+        if (q < T(2)) { T t = T(2) - q; return -sigma / h * T(5) / T(8) * q * (t * t * t); }
+        return T(0);
+    }
+};
+
+template <typename T>
+struct SmoothKernelFn<T, ESPH_WENDLAND_C4> {
+    static constexpr double ratio = 2.0;
+    __device__ static __forceinline__ T value(T r, T h, T sigma) {
+#pragma clang fp contract(off)
+        T q = r / h;
+        if (q < T(2)) {
+            T t = T(2) - q, t2 = t * t, t6 = t2 * t2 * t2;
+            return sigma * t6 * (T(35) * (q * q) + T(36) * q + T(12)) / T(768);
+        }
+        return T(0);
+    }
+    __device__ static __forceinline__ T gradient(T r, T h, T sigma) {
+#pragma clang fp contract(off)
+        T q = r / h;
+        if (q < T(2)) {
+            T t = T(2) - q, t2 = t * t, t5 = t2 * t2 * t;
+            return -sigma / h * (T(35.0 / 96.0) * (q * q) + T(7.0 / 48.0) * q) * t5;
+        }
+        return T(0);
+    }
+};
+
+// Integer power by squaring (UpdatePressure.jl:13-14 uses x^gamma, gamma::Int).
+__device__ __forceinline__ double ipow(double x, int n) {
+    double y = 1.0;
+    while (n > 0) {
+        if (n & 1) y *= x;
+        x *= x;
+        n >>= 1;
+    }
+    return y;
+}
+
+}  // namespace esph

@@ -1,0 +1,40 @@
+// esph_launch.hpp — host-side launchers of the kernels in esph_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "esph_device.hpp"
+
+namespace esph {
+
+// Scenes of one (dim, kernel, precision) class; the pair kernels are
+// compiled per class, so a batch launches one kernel per bucket and phase.
+struct Bucket {
+    int dim;
+    int kernel;
+    int prec;
+    const SceneDev* scenes;  // device table
+    const Tile* tiles;       // device: particle tiles of the bucket's scenes
+    int ntiles;
+};
+
+// Tables for the dimension-agnostic sort kernels of a whole batch.
+struct SortPlan {
+    const SceneDev* scenes;
+    int nscenes;
+    const Tile* ptiles;  // particle tiles (capacity-sized)
+    int nptiles;
+    const Tile* ctiles;  // cell tiles of kCellTile cells
+    int nctiles;
+};
+
+hipError_t launch_continuity(const Bucket& b, hipStream_t st);
+hipError_t launch_forces(const Bucket& b, hipStream_t st);
+hipError_t launch_shepard(const Bucket& b, hipStream_t st);
+hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st);
+hipError_t launch_pairs(const Bucket& b, int pairset, int32_t* counts, const int64_t* offs, int32_t* out,
+                        hipStream_t st);
+hipError_t launch_kernel_eval(int kernel, int64_t n, const double* r, double* w, double* dw, double h,
+                              double sigma, hipStream_t st);
+
+}  // namespace esph

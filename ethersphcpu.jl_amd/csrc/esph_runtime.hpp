@@ -1,0 +1,83 @@
+// esph_runtime.hpp — host-side objects behind the opaque C ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ethersph.h"
+#include "esph_device.hpp"
+
+struct esph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;  // default priority
+    uint64_t budget = 0;           // 0: unlimited
+    uint64_t used = 0;
+    std::string err;
+};
+
+namespace esph {
+
+// Device allocations owned by one object, released together.
+struct Arena {
+    std::vector<void*> ptrs;
+    uint64_t total = 0;
+    void* get(size_t bytes);
+    void release();
+};
+
+struct WallHost {
+    int64_t n = 0;
+    std::vector<double> x, n_, v, gap, T, kappa;
+    std::vector<int32_t> kind;
+};
+
+// Host copies kept alive while their asynchronous uploads are in flight.
+struct WallStage {
+    std::vector<double> x, n_, v, gap, T, kappa;
+    std::vector<int32_t> kind, wid, wcs;
+    std::vector<float> fn, fv, fgap, fT, fkap;
+};
+
+struct TimedStep {
+    hipEvent_t ev[5];
+    bool shepard;
+};
+
+int fail(esph_ctx* ctx, int code, const std::string& msg);
+int hip_fail(esph_ctx* ctx, const char* what, hipError_t e);
+bool kernel_params(int kind, int dim, double R, double* h, double* sigma);
+
+}  // namespace esph
+
+struct esph_scene {
+    esph_ctx* ctx = nullptr;
+    esph_scene_desc d{};
+    double kh = 0, ksigma = 0;
+    esph::SceneDev h{};            // host mirror of the device table entry
+    esph::SceneDev* dev = nullptr; // device table entry (one scene)
+    esph::Tile* ptiles = nullptr;
+    int nptiles = 0;
+    esph::Tile* ctiles = nullptr;
+    int nctiles = 0;
+    esph::Arena arena;
+    esph::WallHost walls;
+    esph::WallStage wall_stage;
+    std::vector<esph::Tile> tile_stage_p, tile_stage_c;
+    bool walls_set = false;
+    bool allocated = false;
+    bool fluid_ready = false;
+    int64_t n_upload = 0;
+    int64_t step = 0;  // host mirror of SceneDyn::step
+    bool timing = false;
+    std::vector<hipEvent_t> timing_events;
+    size_t ev_next = 0;
+    std::vector<esph::TimedStep> timed;
+};
+
+#define HIPCHK(ctx, expr)                                          \
+    do {                                                           \
+        hipError_t e_ = (expr);                                    \
+        if (e_ != hipSuccess) return esph::hip_fail((ctx), #expr, e_); \
+    } while (0)

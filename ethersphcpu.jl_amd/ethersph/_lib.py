@@ -1,0 +1,161 @@
+"""ctypes binding of the C ABI in include/ethersph.h.
+
+The shared library ``libethersph_hip.so`` (built next to this package by
+``make -C ethersphcpu.jl_amd``) is loaded lazily on first use; importing this
+module only defines the ABI structures.  There is no fallback: every compute
+call goes through the HIP library, and a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_uint64, c_void_p
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libethersph_hip.so")
+
+# ---- enums (include/ethersph.h) ------------------------------------------------
+ESPH_OK = 0
+ERRORS = {1: "ESPH_ERR_ARG", 2: "ESPH_ERR_HIP", 3: "ESPH_ERR_BUDGET", 4: "ESPH_ERR_NAN",
+          5: "ESPH_ERR_STATE", 6: "ESPH_ERR_CAPACITY", 7: "ESPH_ERR_NO_DEVICE"}
+
+CUBIC_SPLINE, GAUSSIAN, WENDLAND_C2, WENDLAND_C4 = 0, 1, 2, 3
+SOLVER_FLUID_WALL, SOLVER_FLUID_VELOCITY, SOLVER_FLUID_THERMOSTATIC_WALL, \
+    SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL = 0, 1, 2, 3
+FP64, FP32_BULK = 0, 1
+WALL_COMPULSIVE, WALL_STATIC_VELOCITY, WALL_THERMOSTATIC = 0, 1, 2
+FIELD_X, FIELD_V, FIELD_RHO, FIELD_P, FIELD_C, FIELD_MASS, FIELD_GAP, FIELD_T, FIELD_KAPPA, \
+    FIELD_CP, FIELD_ID = range(11)
+PAIRS_FLUID_FLUID, PAIRS_FLUID_WALL = 0, 1
+
+D3 = c_double * 3
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("dim", c_int32), ("kernel", c_int32), ("influence_radius", c_double),
+        ("solver", c_int32), ("precision", c_int32),
+        ("rho_0", c_double), ("c_0", c_double), ("p_0", c_double), ("gamma", c_int32),
+        ("mu_0", c_double), ("body_force", D3),
+        ("t_0", c_double), ("kappa_0", c_double), ("cp", c_double), ("beta", c_double),
+        ("dt", c_double), ("density_reinitialized_interval", c_int64), ("check_interval", c_int64),
+        ("check_enabled", c_int32), ("check_min", D3), ("check_max", D3),
+        ("fluid_capacity", c_int64), ("domain_set", c_int32), ("domain_min", D3), ("domain_max", D3),
+    ]
+
+
+DP = POINTER(c_double)
+
+
+class FluidArrays(ctypes.Structure):
+    _fields_ = [("n", c_int64), ("x", DP), ("v", DP), ("rho", DP), ("p", DP), ("c", DP),
+                ("mass", DP), ("gap", DP), ("t", DP), ("kappa", DP), ("cp", DP)]
+
+
+class WallArrays(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("n", c_int64), ("x", DP), ("normal", DP), ("gap", DP),
+                ("v", DP), ("t", DP), ("kappa", DP)]
+
+
+class EsphError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libethersph_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} (run make -C {PKG_ROOT})")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = c_void_p
+    sig = {
+        "esph_abi_version": ([], c_int32),
+        "esph_ctx_create": ([c_int32, c_uint64, POINTER(vp)], c_int32),
+        "esph_ctx_destroy": ([vp], c_int32),
+        "esph_last_error": ([vp], c_char_p),
+        "esph_ctx_memory_used": ([vp, POINTER(c_uint64)], c_int32),
+        "esph_kernel_eval": ([vp, c_int32, c_int32, c_double, c_int64, DP, DP, DP], c_int32),
+        "esph_neighbor_list": ([vp, c_int32, c_double, c_int64, DP, c_int64, DP, POINTER(c_int32), c_int64,
+                                POINTER(c_int64)], c_int32),
+        "esph_scene_create": ([vp, POINTER(SceneDesc), POINTER(vp)], c_int32),
+        "esph_scene_destroy": ([vp], c_int32),
+        "esph_scene_set_walls": ([vp, c_int32, POINTER(WallArrays)], c_int32),
+        "esph_scene_set_fluid": ([vp, POINTER(FluidArrays)], c_int32),
+        "esph_scene_step": ([vp, c_int64], c_int32),
+        "esph_scene_sync": ([vp], c_int32),
+        "esph_scene_step_count": ([vp, POINTER(c_int64)], c_int32),
+        "esph_scene_fluid_count": ([vp, POINTER(c_int64)], c_int32),
+        "esph_scene_get_fluid": ([vp, c_int32, DP, c_int64, POINTER(c_int64)], c_int32),
+        "esph_scene_neighbours": ([vp, c_int32, POINTER(c_int32), c_int64, POINTER(c_int64)], c_int32),
+        "esph_scene_enable_timing": ([vp, c_int32], c_int32),
+        "esph_scene_phase_times": ([vp, DP, POINTER(c_int64)], c_int32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def dptr(a: np.ndarray | None):
+    """Pointer to a C-contiguous float64 array (or NULL)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "expected C-contiguous float64"
+    return a.ctypes.data_as(DP)
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device context (esph_ctx): a HIP stream and an HBM budget."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self._L = lib()
+        h = c_void_p()
+        rc = self._L.esph_ctx_create(device, hbm_budget_bytes, ctypes.byref(h))
+        if rc != ESPH_OK:
+            raise EsphError(rc, "esph_ctx_create failed (no GPU?)")
+        self.handle = h
+
+    def check(self, rc: int) -> None:
+        if rc != ESPH_OK:
+            raise EsphError(rc, self._L.esph_last_error(self.handle).decode())
+
+    def memory_used(self) -> int:
+        v = c_uint64()
+        self.check(self._L.esph_ctx_memory_used(self.handle, ctypes.byref(v)))
+        return int(v.value)
+
+    def close(self) -> None:
+        if self.handle:
+            self._L.esph_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx

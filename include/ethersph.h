@@ -1,0 +1,237 @@
+/*
+ * ethersph.h — C ABI of the MI355X (gfx950) per-step SPH particle hot path.
+ *
+ * The library replaces the per-step work of EtherSPHCPU.jl's `solve!` /
+ * `eachStep!` (neighbour search, pair interactions, self updates, Shepard
+ * re-initialisation, particle deletion) for the four fixed-count solvers.
+ * Host code (the reference's Julia types, or the Python mirror in this
+ * repository) packs its particle vectors into contiguous arrays, hands them
+ * over once, steps on the device and reads fields back.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (ESPH_OK == 0).  No C++
+ *    exception crosses this boundary.  esph_last_error() explains the last
+ *    failure of a context (thread-local message per context).
+ *  - Host buffers passed in are borrowed for the duration of the call and
+ *    copied; the library owns all device memory.
+ *  - Vector fields are particle-major: x[i*dim + k], which is the memory
+ *    layout of a Julia Matrix{Float64}(dim, n) (column-major) and of a C
+ *    row-major (n, dim) array.
+ *  - Particle indices are 0-based.  Particle order of downloads is the
+ *    reference's order: upload order, with deleted particles removed
+ *    (stable, as Base.deleteat!).
+ *  - One host thread per context at a time; contexts are independent.
+ *
+ * Neighbour rule (replaces CellListMap.neighborlist; version unpinned, see
+ * Solve/FluidWallSolve.jl:22,26-31): particles i and j are neighbours iff
+ *     d2 <= R*R,  d2 = ((dx0*dx0) + dx1*dx1) [+ dx2*dx2],  dxk = xi_k - xj_k
+ * evaluated in IEEE binary64 with every operation rounded separately (no
+ * fused multiply-add), and i != j for pairs within one set.  r = sqrt(d2).
+ * The rule holds in every precision mode.
+ */
+#ifndef ETHERSPH_H
+#define ETHERSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ESPH_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    ESPH_OK = 0,
+    ESPH_ERR_ARG = 1,      /* invalid argument                           */
+    ESPH_ERR_HIP = 2,      /* HIP runtime error                          */
+    ESPH_ERR_BUDGET = 3,   /* HBM budget of the context exceeded         */
+    ESPH_ERR_NAN = 4,      /* non-finite particle position met in a step */
+    ESPH_ERR_STATE = 5,    /* call out of order (e.g. step before upload)*/
+    ESPH_ERR_CAPACITY = 6, /* output buffer too small                    */
+    ESPH_ERR_NO_DEVICE = 7 /* no usable GPU                              */
+};
+
+/* ---- smoothing kernels: SPHKernel/SmoothKernel/SmoothKernel.jl:15-20 ---- */
+enum {
+    ESPH_CUBIC_SPLINE = 0, /* CubicSpline.jl:39-73  */
+    ESPH_GAUSSIAN = 1,     /* Gaussian.jl:31-56     */
+    ESPH_WENDLAND_C2 = 2,  /* WendlandC2.jl:38-63   */
+    ESPH_WENDLAND_C4 = 3   /* WendlandC4.jl:38-54   */
+};
+
+/* ---- solvers: one per fixed-count eachStep! method under src/Solve/ ------ */
+enum {
+    ESPH_SOLVER_FLUID_WALL = 0,                     /* FluidWallSolve.jl:8-83 */
+    ESPH_SOLVER_FLUID_VELOCITY = 1,                 /* FluidVelocitySolve.jl:8-102 */
+    ESPH_SOLVER_FLUID_THERMOSTATIC_WALL = 2,        /* FluidThermostaticWallSolve.jl:8-125 */
+    ESPH_SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL = 3 /* FluidCompulsiveThermostaticWallSolve.jl:8-152 */
+};
+
+/* ---- arithmetic forms -------------------------------------------------- */
+enum {
+    ESPH_FP64 = 0,     /* everything in binary64 (the reference's form)                 */
+    ESPH_FP32_BULK = 1 /* pair arithmetic and pair sums in binary32; position, velocity,
+                          density and temperature integration and the Shepard sums in
+                          binary64                                                       */
+};
+
+/* ---- wall particle kinds: Particle/FixedParticle/WallParticle.jl ------- */
+enum {
+    ESPH_WALL_COMPULSIVE = 0,      /* CompulsiveWallParticle      :8-19  */
+    ESPH_WALL_STATIC_VELOCITY = 1, /* StaticVelocityWallParticle  :21-34 */
+    ESPH_WALL_THERMOSTATIC = 2     /* ThermostaticWallParticle    :36-51 */
+};
+
+/* ---- downloadable fluid fields (LiquidParticle.jl:8-101) --------------- */
+enum {
+    ESPH_FIELD_X = 0,     /* x_vec_    (dim per particle) */
+    ESPH_FIELD_V = 1,     /* v_vec_    (dim per particle) */
+    ESPH_FIELD_RHO = 2,   /* rho_   */
+    ESPH_FIELD_P = 3,     /* p_     */
+    ESPH_FIELD_C = 4,     /* c_     */
+    ESPH_FIELD_MASS = 5,  /* mass_  */
+    ESPH_FIELD_GAP = 6,   /* gap_   */
+    ESPH_FIELD_T = 7,     /* t_     (thermal) */
+    ESPH_FIELD_KAPPA = 8, /* kappa_ (thermal) */
+    ESPH_FIELD_CP = 9,    /* cp_    (thermal) */
+    ESPH_FIELD_ID = 10    /* upload index of each live particle, as double */
+};
+
+/* ---- neighbour pair sets ---------------------------------------------- */
+enum {
+    ESPH_PAIRS_FLUID_FLUID = 0, /* unordered (min, max) fluid index pairs         */
+    ESPH_PAIRS_FLUID_WALL = 1   /* (fluid index, wall index), walls concatenated
+                                   in the order of their groups                    */
+};
+
+/*
+ * Scene descriptor.  Named fields only: the reference's
+ * DensityReinitializedFixedTimeIntervalForwardEuler positional order
+ * (dt, total_step, output_interval, check_interval,
+ *  density_reinitialized_interval), FixedTimeIntervalDiscretization.jl:10-16,
+ * is easy to swap; here each interval has its own name.
+ */
+typedef struct esph_scene_desc {
+    int32_t dim;              /* 2 or 3 (SmoothKernel.jl:42 also allows 1; not supported) */
+    int32_t kernel;           /* ESPH_CUBIC_SPLINE ...                                  */
+    double influence_radius;  /* R; h = R / radius_ratio (SmoothKernel.jl:43-44)          */
+    int32_t solver;           /* ESPH_SOLVER_*                                          */
+    int32_t precision;        /* ESPH_FP64 / ESPH_FP32_BULK                              */
+    /* CommonWeaklyCompressibleLiquidModel / ThermalWeaklyCompressibleLiquidModel
+       (EquationModel/LiquidModel.jl:10-66); b and nu_0 are derived as there. */
+    double rho_0;
+    double c_0;
+    double p_0;
+    int32_t gamma;
+    double mu_0;
+    double body_force[3];
+    double t_0;               /* thermal only */
+    double kappa_0;           /* thermal only (kept for completeness; unused by the step) */
+    double cp;                /* thermal only */
+    double beta;              /* thermal only */
+    /* time discretisation */
+    double dt;
+    int64_t density_reinitialized_interval; /* <= 0: never */
+    int64_t check_interval;                 /* <= 0: never */
+    /* `check` predicate as data: a fluid particle is deleted at a check step
+       iff for some axis k: x_k > check_max[k] or x_k < check_min[k]
+       (comparisons false for NaN, as the reference's `if x > bound` tests) */
+    int32_t check_enabled;
+    double check_min[3];
+    double check_max[3];
+    /* sizing */
+    int64_t fluid_capacity;   /* 0: number of fluid particles at upload */
+    int32_t domain_set;       /* 1: use domain_min/max for the cell grid, else the
+                                 bounding box of the uploaded particles and the finite
+                                 check bounds, padded by one cell                       */
+    double domain_min[3];
+    double domain_max[3];
+} esph_scene_desc;
+
+typedef struct esph_fluid_arrays {
+    int64_t n;
+    const double* x;      /* n*dim */
+    const double* v;      /* n*dim */
+    const double* rho;    /* n */
+    const double* p;      /* n */
+    const double* c;      /* n */
+    const double* mass;   /* n */
+    const double* gap;    /* n */
+    const double* t;      /* n, thermal solvers (NULL otherwise) */
+    const double* kappa;  /* n, thermal solvers */
+    const double* cp;     /* n, thermal solvers */
+} esph_fluid_arrays;
+
+typedef struct esph_wall_arrays {
+    int32_t kind;         /* ESPH_WALL_* */
+    int64_t n;
+    const double* x;      /* n*dim */
+    const double* normal; /* n*dim */
+    const double* gap;    /* n */
+    const double* v;      /* n*dim, ESPH_WALL_STATIC_VELOCITY (NULL otherwise) */
+    const double* t;      /* n, ESPH_WALL_THERMOSTATIC */
+    const double* kappa;  /* n, ESPH_WALL_THERMOSTATIC */
+} esph_wall_arrays;
+
+typedef struct esph_ctx esph_ctx;
+typedef struct esph_scene esph_scene;
+
+/* ---- library / context -------------------------------------------------- */
+int esph_abi_version(void);
+/* Creates a context on HIP device `device`; hbm_budget_bytes == 0 means no
+   limit.  The context owns one HIP stream (default priority). */
+int esph_ctx_create(int32_t device, uint64_t hbm_budget_bytes, esph_ctx** out);
+int esph_ctx_destroy(esph_ctx* ctx);
+const char* esph_last_error(const esph_ctx* ctx);
+/* Bytes of device memory currently held by scenes of this context. */
+int esph_ctx_memory_used(const esph_ctx* ctx, uint64_t* bytes);
+
+/* ---- kernel evaluation (SmoothKernel.jl:37-48 + kernelValue/kernelGradient)
+   evaluated on the device, for checking the device math ------------------ */
+int esph_kernel_eval(esph_ctx* ctx, int32_t kernel, int32_t dim, double influence_radius,
+                     int64_t n, const double* r, double* w, double* dw);
+
+/* ---- standalone neighbour list: the CellListMap.neighborlist replacement
+   used by findNeighbours! (Neighbour/Neighbour.jl:88-128).
+   ny == 0: pairs within x (unordered, i < j); ny > 0: pairs (i in x, j in y).
+   Writes at most `capacity` pairs as (i, j) int32 to out_pairs (2 per pair)
+   and their count to *m; returns ESPH_ERR_CAPACITY (with *m set) if it does
+   not fit.  Pairs are sorted by (i, j). ------------------------------------ */
+int esph_neighbor_list(esph_ctx* ctx, int32_t dim, double cutoff,
+                       int64_t nx, const double* x, int64_t ny, const double* y,
+                       int32_t* out_pairs, int64_t capacity, int64_t* m);
+
+/* ---- scenes --------------------------------------------------------------- */
+int esph_scene_create(esph_ctx* ctx, const esph_scene_desc* desc, esph_scene** out);
+int esph_scene_destroy(esph_scene* scene);
+/* Wall groups are static; set them (all groups at once) before the fluid. */
+int esph_scene_set_walls(esph_scene* scene, int32_t ngroups, const esph_wall_arrays* groups);
+/* Uploads (or re-uploads) the fluid state and resets the step counter. */
+int esph_scene_set_fluid(esph_scene* scene, const esph_fluid_arrays* fluid);
+/* Advances nsteps steps of eachStep! (stream-ordered, asynchronous). */
+int esph_scene_step(esph_scene* scene, int64_t nsteps);
+/* Waits for the scene's work; reports ESPH_ERR_NAN if a step met a
+   non-finite position. */
+int esph_scene_sync(esph_scene* scene);
+int esph_scene_step_count(const esph_scene* scene, int64_t* step);
+int esph_scene_fluid_count(esph_scene* scene, int64_t* n);
+/* Downloads a fluid field in reference order; capacity in doubles. */
+int esph_scene_get_fluid(esph_scene* scene, int32_t field, double* out, int64_t capacity,
+                         int64_t* written);
+/* Exports the pair set the next step will use (positions as they are now),
+   sorted; same output contract as esph_neighbor_list. */
+int esph_scene_neighbours(esph_scene* scene, int32_t pairset, int32_t* out_pairs,
+                          int64_t capacity, int64_t* m);
+/* Average device time per step of each phase over the steps since the last
+   call (milliseconds): [0] cell sort, [1] continuity+EOS, [2] forces+integrate,
+   [3] Shepard (per step it ran), [4] check; and the number of steps timed.
+   Timing is off unless enabled (it adds events to the stream). */
+int esph_scene_enable_timing(esph_scene* scene, int32_t enable);
+int esph_scene_phase_times(esph_scene* scene, double* ms5, int64_t* steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ETHERSPH_H */
