@@ -47,6 +47,7 @@ def time_gpu(sc, prec, steps, warmup):
     dev.sync()
     wall = time.perf_counter() - t0
     ph = dev.phase_times()
+    ph["stats"] = dev.stats()
     n = dev.fluid_count()
     dev.close()
     return wall, ph, n

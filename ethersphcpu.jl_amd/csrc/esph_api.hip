@@ -162,6 +162,7 @@ extern "C" int esph_scene_create(esph_ctx* ctx, const esph_scene_desc* desc, esp
 }
 
 static void free_scene_device(esph_scene* s) {
+    s->gplan.release();
     if (s->timing_events.size()) {
         for (hipEvent_t e : s->timing_events) (void)hipEventDestroy(e);
         s->timing_events.clear();
@@ -303,6 +304,13 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.h = s->kh;
     hd.sigma = s->ksigma;
     hd.W0 = s->ksigma;
+    hd.inv_h = 1.0 / s->kh;
+    switch (d.kernel) {
+        case ESPH_CUBIC_SPLINE: hd.kA = s->ksigma / 4.0; hd.kB = 3.0 * s->ksigma / (4.0 * s->kh); break;
+        case ESPH_GAUSSIAN: hd.kA = s->ksigma; hd.kB = 2.0 * s->ksigma / s->kh; break;
+        case ESPH_WENDLAND_C2: hd.kA = s->ksigma / 16.0; hd.kB = 5.0 * s->ksigma / (8.0 * s->kh); break;
+        default: hd.kA = s->ksigma / 768.0; hd.kB = s->ksigma / s->kh; break;
+    }
     hd.rho0 = d.rho_0;
     hd.c0 = d.c_0;
     hd.p0 = d.p_0;
@@ -321,18 +329,29 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     const int nw = (int)s->walls.n;
     hd.nw = nw;
     hd.cell_tiles = (hd.ncell + 1 + kCellTile - 1) / kCellTile;
+    // neighbour-list widths: 1.5x the lattice neighbour count at the
+    // particles' spacing (gap of the first particle) plus 16, multiple of 8
+    {
+        double spacing = (f->n > 0 && f->gap && f->gap[0] > 0) ? f->gap[0] : d.influence_radius / 1.5;
+        const double a = d.influence_radius / spacing;
+        const double est = D == 2 ? 3.14159265358979 * a * a : 4.18879020478639 * a * a * a;
+        int k = (int)std::ceil(1.5 * est + 16.0);
+        k = std::min(std::max((k + 7) / 8 * 8, 16), 1024);
+        hd.kff = k;
+        hd.kfw = nw > 0 ? std::min(std::max((k / 2 + 7) / 8 * 8, 16), 1024) : 0;
+    }
 
     // size check against the context budget
     const bool f32 = d.precision == ESPH_FP32_BULK;
     const size_t F = sizeof(float), G = sizeof(double), I = sizeof(int32_t);
-    size_t per_set = (size_t)cap * (2 * D * G + 5 * G + I + (f32 ? F : G)) +
-                     (thermal ? (size_t)cap * 3 * G : 0) +
-                     (f32 ? (size_t)cap * (D * F + 3 * F + (thermal ? 3 * F : 0)) : 0);
+    const size_t P = f32 ? F : G;
+    size_t per_set = (size_t)cap * (2 * D * G + 5 * G + I + 8 * P) + (thermal ? (size_t)cap * (3 * G + P) : 0);
     size_t bytes = 2 * per_set + (size_t)cap * (2 * D * G + G + (thermal ? G : 0) + 3 * I) +
                    (size_t)(hd.ncell + 1) * I * 2 + I + (size_t)hd.cell_tiles * I +
-                   (size_t)nw * (3 * D * G + 3 * G + 2 * I) + (size_t)(hd.ncell + 1) * I +
-                   (f32 ? (size_t)nw * (2 * D * F + 3 * F) : 0) + sizeof(SceneDyn) + sizeof(SceneDev) +
-                   ((size_t)(cap + kBlock - 1) / kBlock + (size_t)hd.cell_tiles) * sizeof(Tile);
+                   (size_t)nw * (3 * D * G + 3 * G + 2 * I + 9 * P) + (size_t)(hd.ncell + 1) * I +
+                   sizeof(SceneDyn) + sizeof(SceneDev) +
+                   ((size_t)(cap + kBlock - 1) / kBlock + (size_t)hd.cell_tiles) * sizeof(Tile) +
+                   (size_t)cap * I * (hd.kff + hd.kfw + 2);
     if (ctx->budget && ctx->used + bytes > ctx->budget)
         return fail(ctx, ESPH_ERR_BUDGET, "scene needs " + std::to_string(bytes) + " bytes; budget left " +
                                               std::to_string(ctx->budget - ctx->used));
@@ -349,22 +368,15 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         S.m = (double*)dal(G * cap);
         S.gap = (double*)dal(G * cap);
         S.id = (int32_t*)dal(I * cap);
-        S.pr2s = dal((f32 ? F : G) * cap);
         if (thermal) {
             S.T = (double*)dal(G * cap);
             S.kappa = (double*)dal(G * cap);
             S.cp = (double*)dal(G * cap);
         }
-        if (f32) {
-            S.vs = dal(F * cap * D);
-            S.ms = dal(F * cap);
-            S.rhos = dal(F * cap);
-            S.gaps = dal(F * cap);
-            if (thermal) { S.Ts = dal(F * cap); S.kappas = dal(F * cap); S.cps = dal(F * cap); }
-        } else {
-            S.vs = S.v; S.ms = S.m; S.rhos = S.rho; S.gaps = S.gap;
-            S.Ts = S.T; S.kappas = S.kappa; S.cps = S.cp;
-        }
+        const size_t P = f32 ? F : G;  // pair precision
+        S.A = dal(4 * P * cap);
+        S.B = dal(4 * P * cap);
+        S.kap = (thermal && D == 3) ? dal(P * cap) : nullptr;
     }
     hd.xn = (double*)dal(G * cap * D);
     hd.vn = (double*)dal(G * cap * D);
@@ -376,6 +388,11 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.count = (int32_t*)dal(I * (hd.ncell + 1));
     hd.cell_start = (int32_t*)dal(I * (hd.ncell + 2));
     hd.bsum = (int32_t*)dal(I * hd.cell_tiles);
+    hd.nbr = (int32_t*)dal(I * (size_t)cap * hd.kff);
+    hd.nnb = (int32_t*)dal(I * (size_t)cap);
+    hd.wnbr = (int32_t*)dal(I * (size_t)cap * std::max(hd.kfw, 1));
+    hd.nnbw = (int32_t*)dal(I * (size_t)cap);
+    hd.ovf = (int32_t*)dal(I * (size_t)cap);
     // walls, binned on the host once (stable by upload index)
     hd.wx = (double*)dal(G * std::max(nw, 1) * D);
     hd.wn = (double*)dal(G * std::max(nw, 1) * D);
@@ -386,14 +403,11 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.wkind = (int32_t*)dal(I * std::max(nw, 1));
     hd.wid = (int32_t*)dal(I * std::max(nw, 1));
     hd.wcell_start = (int32_t*)dal(I * (hd.ncell + 1));
-    if (f32) {
-        hd.wns = dal(F * std::max(nw, 1) * D);
-        hd.wvs = dal(F * std::max(nw, 1) * D);
-        hd.wgaps = dal(F * std::max(nw, 1));
-        hd.wTs = dal(F * std::max(nw, 1));
-        hd.wkappas = dal(F * std::max(nw, 1));
-    } else {
-        hd.wns = hd.wn; hd.wvs = hd.wv; hd.wgaps = hd.wgap; hd.wTs = hd.wT; hd.wkappas = hd.wkappa;
+    {
+        const size_t P = f32 ? F : G;
+        hd.wA = dal(4 * P * std::max(nw, 1));
+        hd.wB = dal(4 * P * std::max(nw, 1));
+        hd.wkap = dal(P * std::max(nw, 1));
     }
     hd.dyn = (SceneDyn*)dal(sizeof(SceneDyn));
     s->dev = (SceneDev*)dal(sizeof(SceneDev));
@@ -401,6 +415,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     s->nctiles = hd.cell_tiles;
     s->ptiles = (Tile*)dal(sizeof(Tile) * s->nptiles);
     s->ctiles = (Tile*)dal(sizeof(Tile) * s->nctiles);
+    s->otiles = (Tile*)dal(sizeof(Tile) * kOvfTiles);
     if (!ok) {
         a.release();
         return fail(ctx, ESPH_ERR_HIP, "scene allocation failed (hipMalloc)");
@@ -442,26 +457,49 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         up(hd.wkind, ws.kind.data(), I * nw);
         up(hd.wid, ws.wid.data(), I * nw);
         up(hd.wcell_start, ws.wcs.data(), I * (hd.ncell + 1));
-        if (f32) {
-            auto tof = [](const std::vector<double>& src) {
-                std::vector<float> o(src.size());
-                for (size_t i = 0; i < src.size(); ++i) o[i] = (float)src[i];
-                return o;
-            };
-            ws.fn = tof(ws.n_); ws.fv = tof(ws.v); ws.fgap = tof(ws.gap); ws.fT = tof(ws.T); ws.fkap = tof(ws.kappa);
-            up(hd.wns, ws.fn.data(), F * nw * D);
-            up(hd.wvs, ws.fv.data(), F * nw * D);
-            up(hd.wgaps, ws.fgap.data(), F * nw);
-            up(hd.wTs, ws.fT.data(), F * nw);
-            up(hd.wkappas, ws.fkap.data(), F * nw);
+        // packed wall records (see SceneDev::wA / wB / wkap)
+        {
+            std::vector<double> A(4 * (size_t)std::max(nw, 1), 0.0), B(4 * (size_t)std::max(nw, 1), 0.0);
+            std::vector<double> K(std::max(nw, 1), 0.0);
+            for (int p = 0; p < nw; ++p) {
+                double* a = &A[4 * (size_t)p];
+                double* b = &B[4 * (size_t)p];
+                if (D == 3) {
+                    a[0] = ws.n_[3 * p]; a[1] = ws.n_[3 * p + 1]; a[2] = ws.n_[3 * p + 2]; a[3] = ws.gap[p];
+                    b[0] = ws.v[3 * p]; b[1] = ws.v[3 * p + 1]; b[2] = ws.v[3 * p + 2]; b[3] = ws.T[p];
+                    K[p] = ws.kappa[p];
+                } else {
+                    a[0] = ws.n_[2 * p]; a[1] = ws.n_[2 * p + 1]; a[2] = ws.gap[p]; a[3] = ws.T[p];
+                    b[0] = ws.v[2 * p]; b[1] = ws.v[2 * p + 1]; b[2] = ws.kappa[p]; b[3] = 0.0;
+                }
+            }
+            if (f32) {
+                ws.fA.assign(A.begin(), A.end());
+                ws.fB.assign(B.begin(), B.end());
+                ws.fK.assign(K.begin(), K.end());
+                up(hd.wA, ws.fA.data(), F * ws.fA.size());
+                up(hd.wB, ws.fB.data(), F * ws.fB.size());
+                up(hd.wkap, ws.fK.data(), F * ws.fK.size());
+            } else {
+                ws.dA = A;
+                ws.dB = B;
+                ws.dK = K;
+                up(hd.wA, ws.dA.data(), G * ws.dA.size());
+                up(hd.wB, ws.dB.data(), G * ws.dB.size());
+                up(hd.wkap, ws.dK.data(), G * ws.dK.size());
+            }
         }
         std::vector<Tile> pt(s->nptiles), ct(s->nctiles);
         for (int t = 0; t < s->nptiles; ++t) pt[t] = Tile{0, t * kBlock};
         for (int t = 0; t < s->nctiles; ++t) ct[t] = Tile{0, t * kCellTile};
+        std::vector<Tile> ot(kOvfTiles);
+        for (int t = 0; t < kOvfTiles; ++t) ot[t] = Tile{0, t * kBlock};
         s->tile_stage_p = pt;
         s->tile_stage_c = ct;
+        s->tile_stage_o = ot;
         up(s->ptiles, s->tile_stage_p.data(), sizeof(Tile) * pt.size());
         up(s->ctiles, s->tile_stage_c.data(), sizeof(Tile) * ct.size());
+        up(s->otiles, s->tile_stage_o.data(), sizeof(Tile) * ot.size());
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return hip_fail(ctx, "wall upload", e);
     }
@@ -530,7 +568,8 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
 }
 
 static Bucket scene_bucket(esph_scene* s) {
-    return Bucket{s->d.dim, s->d.kernel, s->d.precision, s->dev, s->ptiles, s->nptiles};
+    const bool th = (s->h.flags & FLAG_THERMAL) != 0;
+    return Bucket{s->d.dim, s->d.kernel, s->d.precision, s->dev, s->ptiles, s->nptiles, s->otiles, 1, th, !th};
 }
 
 static hipEvent_t timing_event(esph_scene* s) {
@@ -548,6 +587,18 @@ extern "C" int esph_scene_step(esph_scene* s, int64_t nsteps) {
     esph_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
+    if (s->use_graph) {
+        if (s->gplan.nmembers == 0) {
+            HIPCHK(ctx, hipStreamSynchronize(st));
+            s->gplan.use_graph = true;
+            int rc = s->gplan.build(ctx, std::vector<esph_scene*>{s});
+            if (rc) return rc;
+        }
+        int rc = s->gplan.step(ctx, nsteps);
+        if (rc) return rc;
+        s->step += nsteps;
+        return ESPH_OK;
+    }
     const Bucket b = scene_bucket(s);
     const SortPlan plan{s->dev, 1, s->ptiles, s->nptiles, s->ctiles, s->nctiles};
     const int64_t K = s->d.density_reinitialized_interval;
@@ -800,5 +851,29 @@ extern "C" int esph_scene_phase_times(esph_scene* s, double* ms5, int64_t* steps
     if (steps) *steps = n;
     s->timed.clear();
     s->ev_next = 0;
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_stats(esph_scene* s, int64_t* out4) {
+    if (!s || !out4) return ESPH_ERR_ARG;
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    SceneDyn dyn;
+    int rc = read_dyn(s, &dyn);
+    if (rc) return rc;
+    out4[0] = dyn.overflow;
+    out4[1] = s->h.kff;
+    out4[2] = s->h.kfw;
+    out4[3] = s->h.ncell;
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_enable_graph(esph_scene* s, int32_t enable) {
+    if (!s) return ESPH_ERR_ARG;
+    s->use_graph = enable != 0;
+    if (!s->use_graph) {
+        (void)hipSetDevice(s->ctx->device);
+        (void)hipStreamSynchronize(s->ctx->stream);
+        s->gplan.release();
+    }
     return ESPH_OK;
 }

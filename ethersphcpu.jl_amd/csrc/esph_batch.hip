@@ -1,0 +1,197 @@
+// esph_batch.hip — batched stepping of many resident scenes.
+//
+// A batch holds a device table of the member scenes' SceneDev entries and
+// tile tables that map thread blocks to (scene, first particle).  One step of
+// the batch is one launch per (dim, kernel, precision) bucket and phase plus
+// one sort launch for all members, so small scenes share launches instead of
+// paying them one by one.  Optionally the step is captured once as a HIP graph
+// and replayed.  Members keep their own device state (SceneDyn): a scene
+// stepped alone or inside a batch runs the same per-particle arithmetic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ethersph.h"
+#include "esph_device.hpp"
+#include "esph_launch.hpp"
+#include "esph_runtime.hpp"
+
+using namespace esph;
+
+namespace esph {
+
+int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
+    release();
+    const int n = (int)members.size();
+    std::vector<SceneDev> table(n);
+    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+    std::vector<Tile> all_p, all_c;
+    for (int k = 0; k < n; ++k) {
+        const esph_scene* s = members[k];
+        table[k] = s->h;
+        groups[{s->d.dim, s->d.kernel, s->d.precision}].push_back(k);
+        for (int t = 0; t < s->nptiles; ++t) all_p.push_back(Tile{k, t * kBlock});
+        for (int t = 0; t < s->nctiles; ++t) all_c.push_back(Tile{k, t * kCellTile});
+    }
+    // per-bucket tile tables, laid out after the global ones in one buffer
+    std::vector<Tile> tiles(all_p);
+    tiles.insert(tiles.end(), all_c.begin(), all_c.end());
+    struct Off { size_t p, np, o, no; bool th, pl; int dim, kernel, prec, ns; };
+    std::vector<Off> offs;
+    for (auto& g : groups) {
+        Off o{};
+        o.dim = std::get<0>(g.first);
+        o.kernel = std::get<1>(g.first);
+        o.prec = std::get<2>(g.first);
+        o.ns = (int)g.second.size();
+        o.p = tiles.size();
+        for (int k : g.second) {
+            for (int t = 0; t < members[k]->nptiles; ++t) tiles.push_back(Tile{k, t * kBlock});
+            const bool th = (members[k]->h.flags & FLAG_THERMAL) != 0;
+            o.th = o.th || th;
+            o.pl = o.pl || !th;
+        }
+        o.np = tiles.size() - o.p;
+        o.o = tiles.size();
+        for (int k : g.second)
+            for (int t = 0; t < kOvfTiles; ++t) tiles.push_back(Tile{k, t * kBlock});
+        o.no = tiles.size() - o.o;
+        offs.push_back(o);
+    }
+    SceneDev* dtable = (SceneDev*)arena.get(sizeof(SceneDev) * std::max(n, 1));
+    Tile* dtiles = (Tile*)arena.get(sizeof(Tile) * std::max<size_t>(tiles.size(), 1));
+    if (!dtable || !dtiles) {
+        release();
+        return fail(ctx, ESPH_ERR_HIP, "batch allocation failed");
+    }
+    hipError_t e = hipMemcpy(dtable, table.data(), sizeof(SceneDev) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !tiles.empty())
+        e = hipMemcpy(dtiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(ctx, "batch upload", e);
+    }
+    buckets.clear();
+    for (const Off& o : offs)
+        buckets.push_back(Bucket{o.dim, o.kernel, o.prec, dtable, dtiles + o.p, (int)o.np, dtiles + o.o, o.ns, o.th,
+                                 o.pl});
+    plan = SortPlan{dtable, n, dtiles, (int)all_p.size(), dtiles + all_p.size(), (int)all_c.size()};
+    nmembers = n;
+    return ESPH_OK;
+}
+
+hipError_t StepPlan::launch_step(hipStream_t st) const {
+    hipError_t e = hipSuccess;
+    for (const Bucket& b : buckets)
+        if (e == hipSuccess) e = launch_continuity(b, st);
+    for (const Bucket& b : buckets)
+        if (e == hipSuccess) e = launch_forces(b, st);
+    for (const Bucket& b : buckets)  // gated per scene on the device
+        if (e == hipSuccess) e = launch_shepard(b, st);
+    if (e == hipSuccess) e = launch_sort(plan, 0, st);
+    return e;
+}
+
+int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
+    hipStream_t st = ctx->stream;
+    if (use_graph && !exec && nsteps > 0) {
+        hipGraph_t g = nullptr;
+        HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = launch_step(st);
+        hipError_t e2 = hipStreamEndCapture(st, &g);
+        if (e != hipSuccess) return hip_fail(ctx, "graph capture", e);
+        if (e2 != hipSuccess) return hip_fail(ctx, "graph capture end", e2);
+        graph = g;
+        HIPCHK(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    for (int64_t t = 0; t < nsteps; ++t) {
+        hipError_t e = use_graph ? hipGraphLaunch(exec, st) : launch_step(st);
+        if (e != hipSuccess) return hip_fail(ctx, "batch step", e);
+    }
+    return ESPH_OK;
+}
+
+void StepPlan::release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    arena.release();
+    buckets.clear();
+    nmembers = 0;
+}
+
+}  // namespace esph
+
+struct esph_batch {
+    esph_ctx* ctx = nullptr;
+    std::vector<esph_scene*> members;
+    esph::StepPlan plan;
+};
+
+extern "C" int esph_batch_create(esph_ctx* ctx, int32_t nscenes, esph_scene* const* scenes, esph_batch** out) {
+    if (!ctx || !out || nscenes <= 0 || !scenes) return fail(ctx, ESPH_ERR_ARG, "batch_create: bad argument");
+    *out = nullptr;
+    std::vector<esph_scene*> m(scenes, scenes + nscenes);
+    for (esph_scene* s : m) {
+        if (!s || s->ctx != ctx) return fail(ctx, ESPH_ERR_ARG, "batch_create: scene of another context");
+        if (!s->fluid_ready) return fail(ctx, ESPH_ERR_STATE, "batch_create: scene without fluid");
+    }
+    std::vector<esph_scene*> u(m);
+    std::sort(u.begin(), u.end());
+    if (std::adjacent_find(u.begin(), u.end()) != u.end()) return fail(ctx, ESPH_ERR_ARG, "batch_create: duplicate scene");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    esph_batch* b = new esph_batch();
+    b->ctx = ctx;
+    b->members = m;
+    int rc = b->plan.build(ctx, m);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return ESPH_OK;
+}
+
+extern "C" int esph_batch_destroy(esph_batch* b) {
+    if (!b) return ESPH_ERR_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    b->plan.release();
+    delete b;
+    return ESPH_OK;
+}
+
+extern "C" int esph_batch_enable_graph(esph_batch* b, int32_t enable) {
+    if (!b) return ESPH_ERR_ARG;
+    if (!enable && b->plan.exec) {
+        (void)hipGraphExecDestroy(b->plan.exec);
+        (void)hipGraphDestroy(b->plan.graph);
+        b->plan.exec = nullptr;
+        b->plan.graph = nullptr;
+    }
+    b->plan.use_graph = enable != 0;
+    return ESPH_OK;
+}
+
+extern "C" int esph_batch_step(esph_batch* b, int64_t nsteps) {
+    if (!b || nsteps < 0) return fail(b ? b->ctx : nullptr, ESPH_ERR_ARG, "batch_step: bad argument");
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    int rc = b->plan.step(b->ctx, nsteps);
+    if (rc) return rc;
+    for (esph_scene* s : b->members) s->step += nsteps;
+    return ESPH_OK;
+}
+
+extern "C" int esph_batch_sync(esph_batch* b) {
+    if (!b) return ESPH_ERR_ARG;
+    for (esph_scene* s : b->members) {
+        int rc = esph_scene_sync(s);
+        if (rc) return rc;
+    }
+    return ESPH_OK;
+}

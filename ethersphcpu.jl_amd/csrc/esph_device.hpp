@@ -13,6 +13,7 @@ namespace esph {
 constexpr int kBlock = 256;          // particles per tile == threads per block
 constexpr int kScanItems = 4;        // cells per thread in the scan kernels
 constexpr int kCellTile = kBlock * kScanItems;
+constexpr int kOvfTiles = 16;        // tiles per scene of the overflow (cell-scan) kernels
 
 // SceneDev::flags
 enum : int {
@@ -28,12 +29,18 @@ struct SceneDyn {
     int32_t parity;  // which FluidSet is current
     int32_t n_live;  // fluid particles in the current set
     int32_t err;     // bit 0: a non-finite position was met
+    int32_t overflow;  // particle-steps whose neighbour count exceeded the list width
+    int32_t novf;      // this step's overflowed particles (listed in SceneDev::ovf)
     int32_t pad;
 };
 
-// One sorted copy of the fluid state.  Masters are binary64; the *s pointers
-// are what the pair loops read: binary32 shadows in FP32_BULK, the masters
-// themselves in FP64.
+// One sorted copy of the fluid state.  Masters are binary64 SoA arrays (state
+// integration and downloads).  A, B (and kap) are the packed records the
+// pair loops gather, in the pair precision T (float in FP32_BULK, double in
+// FP64), 4 values per particle so one record is one or two vector loads:
+//   3D: A = (vx, vy, vz, m)   B = (rho, p/rho^2, gap, T)   kap = kappa
+//   2D: A = (vx, vy, m, gap)  B = (rho, p/rho^2, T, kappa)
+// The sort writes A, B.zw and kap; the continuity pass writes B.xy.
 struct FluidSet {
     double* x;      // n*dim
     double* v;      // n*dim
@@ -46,14 +53,9 @@ struct FluidSet {
     double* kappa;
     double* cp;
     int32_t* id;    // upload index
-    void* vs;       // n*dim
-    void* ms;
-    void* rhos;
-    void* pr2s;     // p / rho^2 (pressure term precomputed per particle)
-    void* gaps;
-    void* Ts;
-    void* kappas;
-    void* cps;
+    void* A;
+    void* B;
+    void* kap;
 };
 
 struct SceneDev {
@@ -66,8 +68,10 @@ struct SceneDev {
     int32_t nw;          // wall particles
     int64_t kreinit;     // density_reinitialized_interval (<=0: never)
     int64_t kcheck;      // check_interval (<=0: never)
-    // kernel (SmoothKernel.jl:37-48)
+    // kernel (SmoothKernel.jl:37-48); kA, kB: factors of W(q) and dW/dr(q)
+    // in the q-forms below, inv_h = 1 / h
     double R, R2, h, sigma, W0;
+    double inv_h, kA, kB;
     // model (LiquidModel.jl:10-66)
     double rho0, c0, p0, b, nu0, t0, cp, beta, dt;
     double g[3];
@@ -88,7 +92,15 @@ struct SceneDev {
     int32_t* tmp;         // cap
     int32_t* bsum;        // scan block sums
     int32_t cell_tiles;   // ceil((ncell+1) / kCellTile)
+    int32_t kff;          // neighbour-list width (fluid-fluid)
+    int32_t kfw;          // neighbour-list width (fluid-wall)
     int32_t pad0;
+    // per-step neighbour lists, transposed: entry k of particle i at [k * cap + i]
+    int32_t* nbr;
+    int32_t* nnb;         // fluid-fluid count per particle
+    int32_t* wnbr;
+    int32_t* nnbw;        // fluid-wall count per particle
+    int32_t* ovf;         // particles whose lists overflowed this step (they scan their cells)
     // walls: static, sorted by cell once
     double* wx;      // nw*dim
     double* wn;      // nw*dim
@@ -99,12 +111,24 @@ struct SceneDev {
     int32_t* wkind;
     int32_t* wid;    // wall index in group-concatenated upload order
     int32_t* wcell_start;  // ncell + 1
-    void* wgaps;     // pair-precision views
-    void* wns;
-    void* wvs;
-    void* wTs;
-    void* wkappas;
+    // packed wall records in the pair precision (static):
+    //   3D: wA = (nx, ny, nz, gap)  wB = (vx, vy, vz, T)  wkap = kappa
+    //   2D: wA = (nx, ny, gap, T)   wB = (vx, vy, kappa, 0)
+    void* wA;
+    void* wB;
+    void* wkap;
     SceneDyn* dyn;
+};
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+    using type = float4;
+};
+template <>
+struct Vec4<double> {
+    using type = double4;
 };
 
 struct Tile {
@@ -241,6 +265,87 @@ struct SmoothKernelFn<T, ESPH_WENDLAND_C4> {
         return T(0);
     }
 };
+
+// ---------------------------------------------------------------------------
+// The same kernels as functions of q = r / h with the leading factors folded
+// into kA and kB (pair loops: no division per kernel evaluation):
+//   CubicSpline  kA = sigma/4,   kB = 3 sigma / (4 h)
+//   Gaussian     kA = sigma,     kB = 2 sigma / h
+//   WendlandC2   kA = sigma/16,  kB = 5 sigma / (8 h)
+//   WendlandC4   kA = sigma/768, kB = sigma / h
+// They equal the reference-order forms above up to rounding.
+// ---------------------------------------------------------------------------
+template <typename T, int K>
+struct KernelQ;
+
+template <typename T>
+struct KernelQ<T, ESPH_CUBIC_SPLINE> {
+    __device__ static __forceinline__ void wdw(T q, T kA, T kB, T& w, T& dw) {
+        if (q < T(1)) {
+            w = kA * (T(3) * q * q * (q - T(2)) + T(4));
+            dw = kB * q * (T(3) * q - T(4));
+        } else if (q < T(2)) {
+            const T t = T(2) - q;
+            w = kA * t * t * t;
+            dw = -kB * t * t;
+        } else {
+            w = T(0);
+            dw = T(0);
+        }
+    }
+};
+
+template <typename T>
+struct KernelQ<T, ESPH_GAUSSIAN> {
+    __device__ static __forceinline__ void wdw(T q, T kA, T kB, T& w, T& dw) {
+        if (q < T(3)) {
+            const T e = exp(-(q * q));
+            w = kA * e;
+            dw = -kB * q * e;
+        } else {
+            w = T(0);
+            dw = T(0);
+        }
+    }
+};
+
+template <typename T>
+struct KernelQ<T, ESPH_WENDLAND_C2> {
+    __device__ static __forceinline__ void wdw(T q, T kA, T kB, T& w, T& dw) {
+        if (q < T(2)) {
+            const T t = T(2) - q, t3 = t * t * t;
+            w = kA * t3 * t * (T(1) + T(2) * q);
+            dw = -kB * q * t3;
+        } else {
+            w = T(0);
+            dw = T(0);
+        }
+    }
+};
+
+template <typename T>
+struct KernelQ<T, ESPH_WENDLAND_C4> {
+    __device__ static __forceinline__ void wdw(T q, T kA, T kB, T& w, T& dw) {
+        if (q < T(2)) {
+            const T t = T(2) - q, t2 = t * t, t5 = t2 * t2 * t;
+            w = kA * t5 * t * (T(35) * q * q + T(36) * q + T(12));
+            dw = -kB * (T(35.0 / 96.0) * q * q + T(7.0 / 48.0) * q) * t5;
+        } else {
+            w = T(0);
+            dw = T(0);
+        }
+    }
+};
+
+// Reciprocal, square root and reciprocal square root of the pair arithmetic:
+// IEEE in binary64; the hardware approximations (v_rcp_f32, v_sqrt_f32,
+// v_rsq_f32, ~1 ulp) in binary32.
+__device__ __forceinline__ double prcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float prcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double psqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float psqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double prsqrt(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float prsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Integer power by squaring (UpdatePressure.jl:13-14 uses x^gamma, gamma::Int).
 __device__ __forceinline__ double ipow(double x, int n) {

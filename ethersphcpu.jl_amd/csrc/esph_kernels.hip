@@ -13,6 +13,8 @@
 #include "esph_device.hpp"
 #include "esph_launch.hpp"
 
+#include <type_traits>
+
 namespace esph {
 
 template <typename T>
@@ -34,7 +36,8 @@ __device__ __forceinline__ void load_x(const double* __restrict__ x, int j, doub
 }
 
 // Iterates the particles of cells adjacent to cell c (3 x-contiguous ranges
-// in 2D, 9 in 3D) whose distance to xi satisfies the neighbour rule.
+// in 2D, 9 in 3D) whose distance to xi satisfies the neighbour rule.  The
+// candidate loop issues the loads of four candidates before testing them.
 template <int D, typename F>
 __device__ __forceinline__ void for_each_neighbour(const SceneDev& s, const int32_t* __restrict__ cstart,
                                                    const double* __restrict__ xall, const double* xi,
@@ -42,6 +45,7 @@ __device__ __forceinline__ void for_each_neighbour(const SceneDev& s, const int3
     const int zlo = D == 3 ? -1 : 0, zhi = D == 3 ? 1 : 0;
     const int cx0 = c[0] > 0 ? c[0] - 1 : 0;
     const int cx1 = c[0] + 1 < s.nc[0] ? c[0] + 1 : s.nc[0] - 1;
+    const double R2 = s.R2;
     for (int dz = zlo; dz <= zhi; ++dz) {
         const int cz = c[2] + dz;
         if (cz < 0 || cz >= s.nc[2]) continue;
@@ -50,55 +54,208 @@ __device__ __forceinline__ void for_each_neighbour(const SceneDev& s, const int3
             if (cy < 0 || cy >= s.nc[1]) continue;
             const int j0 = cstart[cell_linear(s, cx0, cy, cz)];
             const int j1 = cstart[cell_linear(s, cx1, cy, cz) + 1];
-            for (int j = j0; j < j1; ++j) {
-                if (j == self) continue;
-                double xj[3], rv[3];
-                load_x<D>(xall, j, xj);
-                const double d2 = dist2_exact<D>(xi, xj, rv);
-                if (d2 <= s.R2) f(j, d2, rv);
+            for (int j = j0; j < j1; j += 4) {
+                double xj[4][3];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) load_x<D>(xall, j + u < j1 ? j + u : j1 - 1, xj[u]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = j + u;
+                    double rv[3];
+                    const double d2 = dist2_exact<D>(xi, xj[u], rv);
+                    if (jj < j1 && jj != self && d2 <= R2) f(jj, d2, rv);
+                }
             }
         }
     }
 }
 
+// Records the neighbours of particle i (pair rule of for_each_neighbour) in
+// the transposed list `list` (entry k at [k * cap + i]); returns the count.
+// Entries beyond `width` are dropped and the particle later scans its cells.
+template <int D>
+__device__ __forceinline__ int build_list(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                          const double* __restrict__ xall, const double* xi, const int c[3],
+                                          int i, int self, int32_t* __restrict__ list, int width) {
+    int cnt = 0;
+    for_each_neighbour<D>(s, cstart, xall, xi, c, self, [&](int j, double, const double*) {
+        if (cnt < width) list[(size_t)cnt * s.cap + i] = j;
+        ++cnt;
+    });
+    return cnt;
+}
+
+// Iterates the recorded neighbours of particle i with the same (j, d2, rv)
+// arguments as for_each_neighbour; the next entry's index and position are
+// loaded while the current one is processed.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_listed(const SceneDev& s, const double* __restrict__ xall,
+                                                const double* xi, int i, const int32_t* __restrict__ list,
+                                                int cnt, F&& f) {
+    if (cnt <= 0) return;
+    const size_t cap = (size_t)s.cap;
+    int jn = list[i];
+    double xn[3];
+    load_x<D>(xall, jn, xn);
+    for (int k = 0; k < cnt; ++k) {
+        const int j = jn;
+        double xj[3] = {xn[0], xn[1], D == 3 ? xn[2] : 0.0};
+        if (k + 1 < cnt) {
+            jn = list[(size_t)(k + 1) * cap + i];
+            load_x<D>(xall, jn, xn);
+        }
+        double rv[3];
+        const double d2 = dist2_exact<D>(xi, xj, rv);
+        f(j, d2, rv);
+    }
+}
+
+// Pairs of particle i: from the step's list, or (SCAN, for particles whose
+// list overflowed) by scanning the cells.  Both visit the identical pair set.
+template <int D, bool SCAN, typename F>
+__device__ __forceinline__ void for_pairs(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                          const double* __restrict__ xall, const double* xi, int i, int self,
+                                          const int32_t* __restrict__ list, int cnt, F&& f) {
+    if (SCAN) {
+        int c[3];
+        cell_of<D>(s, xi, c);
+        for_each_neighbour<D>(s, cstart, xall, xi, c, self, f);
+    } else {
+        for_each_listed<D>(s, xall, xi, i, list, cnt, f);
+    }
+}
+
+// The SCAN instances walk the step's overflow list with a small fixed grid:
+// kOvfTiles tiles of kBlock entries per scene, striding over the list.
+// Calls f(i) for the particle(s) of this thread: the tile slot (main
+// kernels) or, in the SCAN instances, every kOvfTiles*kBlock-th entry of the
+// step's overflow list.
+template <bool SCAN, typename F>
+__device__ __forceinline__ void for_work(const SceneDev& s, const SceneDyn& dyn, const Tile& tile, F&& f) {
+    const int q0 = tile.begin + (int)threadIdx.x;
+    if (SCAN) {
+        for (int q = q0; q < dyn.novf; q += kOvfTiles * kBlock) f(s.ovf[q]);
+    } else if (q0 < dyn.n_live) {
+        f(q0);
+    }
+}
+
+__device__ __forceinline__ bool overflowed(const SceneDev& s, int i) {
+    return s.nnb[i] > s.kff || (s.nw > 0 && s.nnbw[i] > s.kfw);
+}
+
+// Unpacked neighbour record (see FluidSet).
+template <typename T>
+struct Rec {
+    T v[3];
+    T m, rho, pr2, gap, Tt, kap;
+};
+
+template <int D, typename T>
+__device__ __forceinline__ void load_A(const void* A, int j, T* v, T& m, T& gap) {
+    using V = typename Vec4<T>::type;
+    const V a = static_cast<const V*>(A)[j];
+    v[0] = a.x;
+    v[1] = a.y;
+    if (D == 3) {
+        v[2] = a.z;
+        m = a.w;
+    } else {
+        m = a.z;
+        gap = a.w;
+    }
+}
+
+template <int D, typename T, bool THERMAL>
+__device__ __forceinline__ void load_rec(const FluidSet& S, int j, Rec<T>& r) {
+    using V = typename Vec4<T>::type;
+    load_A<D, T>(S.A, j, r.v, r.m, r.gap);
+    const V b = static_cast<const V*>(S.B)[j];
+    r.rho = b.x;
+    r.pr2 = b.y;
+    if (D == 3) {
+        r.gap = b.z;
+        r.Tt = b.w;
+        if (THERMAL) r.kap = ld<T>(S.kap, j);
+    } else {
+        r.Tt = b.z;
+        r.kap = b.w;
+    }
+}
+
+// Record fields written by the sort (A, B.zw, kap) for particle p.
+template <typename T>
+__device__ __forceinline__ void store_sorted_rec(const FluidSet& O, int D, int p, const double* v, double m,
+                                                 double gap, double Tt, double kap, bool thermal) {
+    T* A = static_cast<T*>(O.A) + 4 * (size_t)p;
+    T* B = static_cast<T*>(O.B) + 4 * (size_t)p;
+    if (D == 3) {
+        A[0] = (T)v[0]; A[1] = (T)v[1]; A[2] = (T)v[2]; A[3] = (T)m;
+        B[2] = (T)gap;
+        B[3] = (T)Tt;
+        if (thermal) static_cast<T*>(O.kap)[p] = (T)kap;
+    } else {
+        A[0] = (T)v[0]; A[1] = (T)v[1]; A[2] = (T)m; A[3] = (T)gap;
+        B[2] = (T)Tt;
+        B[3] = (T)kap;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // continuity! (Interaction/Continuity.jl:8-17), pull form:
-//   drho_i = sum_j m_j (v_i - v_j) . gradW_ij
+//   drho_i = sum_j m_j (v_i - v_j) . gradW_ij,  gradW_ij = W'(r) r_ij / r
 // then updateDensity! (SelfAction/UpdateDensity.jl:8-12) and updatePressure!
-// (SelfAction/UpdatePressure.jl:8-16).  Writes rho, p, c and p/rho^2.
+// (SelfAction/UpdatePressure.jl:8-16).  Writes rho, p, c and the record's
+// (rho, p/rho^2).  This pass also records the step's neighbour lists
+// (positions of the step start), which the force and Shepard passes reuse.
 // ---------------------------------------------------------------------------
-template <int D, int K, typename T>
+template <int D, int K, typename T, bool SCAN>
 __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles, int ntiles) {
-    using SK = SmoothKernelFn<T, K>;
+    using KQ = KernelQ<T, K>;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
-        const int i = tile.begin + (int)threadIdx.x;
-        if (i >= dyn.n_live) continue;
+        for_work<SCAN>(s, dyn, tile, [&](int i) {
         const FluidSet& S = s.set[dyn.parity];
-        const T h = (T)s.h, sigma = (T)s.sigma;
+        const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
         double xi[3];
         load_x<D>(S.x, i, xi);
-        T vi[3];
-#pragma unroll
-        for (int k = 0; k < D; ++k) vi[k] = ld<T>(S.vs, i * D + k);
-        int c[3];
-        cell_of<D>(s, xi, c);
+        T vi[3], m_i, gap_i;
+        load_A<D, T>(S.A, i, vi, m_i, gap_i);
+        int cnt = 0;
+        if (!SCAN) {
+            int c[3];
+            cell_of<D>(s, xi, c);
+            cnt = build_list<D>(s, s.cell_start, S.x, xi, c, i, i, s.nbr, s.kff);
+            s.nnb[i] = cnt;
+            bool over = cnt > s.kff;
+            if (s.nw > 0) {
+                const int cw = build_list<D>(s, s.wcell_start, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
+                s.nnbw[i] = cw;
+                over = over || cw > s.kfw;
+            }
+            if (over) {
+                // finished by the SCAN instance of this kernel
+                s.ovf[atomicAdd(&s.dyn->novf, 1)] = i;
+                atomicAdd(&s.dyn->overflow, 1);
+                return;
+            }
+        }
         T drho = T(0);
-        for_each_neighbour<D>(s, s.cell_start, S.x, xi, c, i, [&](int j, double d2, const double* rv) {
-            const double r = sqrt(d2);
-            const T dW = SK::gradient((T)r, h, sigma);
-            const T inv = T(1) / (T)r;
+        for_pairs<D, SCAN>(s, s.cell_start, S.x, xi, i, i, s.nbr, cnt, [&](int j, double d2, const double* rv) {
+            T vj[3], m_j, g_j;
+            load_A<D, T>(S.A, j, vj, m_j, g_j);
+            const T r = psqrt((T)d2);
+            T W, dW;
+            KQ::wdw(r * inv_h, kA, kB, W, dW);
+            // normalize(0) is NaN and the reference zeroes it (Neighbour.jl:54-59)
+            const T sg = d2 == 0.0 ? T(0) : dW * prcp(r);
             T dot = T(0);
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                T g = dW * ((T)rv[k] * inv);
-                if (r == 0.0) g = T(0);  // normalize(0) is NaN -> 0 (Neighbour.jl:54-59)
-                dot = dot + (vi[k] - ld<T>(S.vs, j * D + k)) * g;
-            }
-            drho += dot * ld<T>(S.ms, j);
+            for (int k = 0; k < D; ++k) dot += (vi[k] - vj[k]) * (T)rv[k];
+            drho += m_j * sg * dot;
         });
         {
 #pragma clang fp contract(off)
@@ -110,14 +267,16 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
             S.rho[i] = rho;
             S.p[i] = p;
             S.c[i] = cc;
-            static_cast<T*>(S.pr2s)[i] = (T)(p / (rho * rho));
-            if (sizeof(T) != sizeof(double)) static_cast<T*>(S.rhos)[i] = (T)rho;
+            T* B = static_cast<T*>(S.B) + 4 * (size_t)i;
+            B[0] = (T)rho;
+            B[1] = (T)(p / (rho * rho));
         }
+        });
     }
 }
 
 // ---------------------------------------------------------------------------
-// Momentum (and heat) pass + integration.
+// Momentum (and heat) pass + integration, pull form over the step's lists.
 //  FF: pressureForce! (PressureForce.jl:8-21), viscosityForce!
 //      (ViscosityForce.jl:8-25), thermalConduction! (ThermalConduction.jl:8-22)
 //  FW: wallForce! (WallForce.jl:8-30), pressureForce! wall
@@ -126,188 +285,189 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
 //      thermostat (ThermalConduction.jl:24-36)
 //  then updateTemperature!, updateVelocity!(…, body force), updatePosition!
 //  (UpdateTemperature.jl:8-12, UpdateVelocity.jl:8-23, UpdatePosition.jl:8-14).
+// The pair terms are the reference's, regrouped so that each pair needs at
+// most two reciprocals:
+//   viscosity  4 (mu_i + mu_j) r W' / (rho_i + rho_j)^2 / (r^2 + 0.01 h^2)
+//              = 4 nu0 r W' / ((rho_i + rho_j) (r^2 + 0.01 h^2))
+//   conduction 2 k_eq r W' (T_i - T_j) / rho_i / rho_j / (r^2 + .01 h^2) m_j / cp_i
+//              = 4 k_i k_j r W' (T_i - T_j) m_j / ((k_i + k_j) rho_j (r^2 + .01 h^2)) / (rho_i cp_i)
+// and 1 / W(gap) is taken per particle when both gaps are equal.
 // New x, v, T go to xn, vn, Tn in the current particle order.
 // ---------------------------------------------------------------------------
-template <int D, int K, typename T>
+template <int D, int K, typename T, bool THERMAL, bool SCAN>
+__device__ __forceinline__ void forces_body(const SceneDev& s, const FluidSet& S, int i) {
+    using KQ = KernelQ<T, K>;
+    const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
+    const T eta2 = (T)(0.01 * (s.h * s.h));
+    const T nu0 = (T)s.nu0;
+    double xi[3];
+    load_x<D>(S.x, i, xi);
+    Rec<T> me;
+    load_rec<D, T, THERMAL>(S, i, me);
+    const T rho_i = me.rho, pr2_i = me.pr2, gap_i = me.gap, m_i = me.m, T_i = me.Tt, kap_i = me.kap;
+    T W_gap, dW_gap;
+    KQ::wdw(gap_i * inv_h, kA, kB, W_gap, dW_gap);
+    const T inv_Wgap_i = prcp(W_gap);
+    T dv[3] = {T(0), T(0), T(0)};
+    T dT = T(0);
+    for_pairs<D, SCAN>(s, s.cell_start, S.x, xi, i, i, s.nbr, s.nnb[i], [&](int j, double d2, const double* rv) {
+        Rec<T> o;
+        load_rec<D, T, THERMAL>(S, j, o);
+        const T r = psqrt((T)d2);
+        T W, dW;
+        KQ::wdw(r * inv_h, kA, kB, W, dW);
+        const T sg = d2 == 0.0 ? T(0) : dW * prcp(r);
+        // pressure, with the 1% correction W(r) / W((gap_i + gap_j) / 2)
+        T P = pr2_i + o.pr2;
+        T inv_Wg = inv_Wgap_i;
+        if (o.gap != gap_i) {
+            T w2, d2w;
+            KQ::wdw((gap_i + o.gap) * T(0.5) * inv_h, kA, kB, w2, d2w);
+            inv_Wg = prcp(w2);
+        }
+        P += fabs(P) * T(0.01) * W * inv_Wg;
+        // viscosity
+        const T r2e = r * r + eta2;
+        const T vf = T(4) * nu0 * r * dW * prcp((rho_i + o.rho) * r2e);
+        const T ps = -P * sg;
+#pragma unroll
+        for (int k = 0; k < D; ++k) dv[k] += o.m * (ps * (T)rv[k] + vf * (me.v[k] - o.v[k]));
+        if (THERMAL)
+            dT += T(4) * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e);
+    });
+    if (THERMAL) dT *= prcp(rho_i * (T)S.cp[i]);
+    if (s.nw > 0) {
+        using V = typename Vec4<T>::type;
+        const bool wall_pressure = (s.flags & FLAG_WALL_PRESSURE) != 0;
+        const T c_i = (T)S.c[i];
+        const T P0 = (T)(s.p0 / (s.rho0 * s.rho0));
+        const T inv_rho_i = prcp(rho_i);
+        const T th_w = (T)(1.0 / (s.cp * s.rho0)) * inv_rho_i * m_i;
+        T dTw = T(0);
+        for_pairs<D, SCAN>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], [&](int j, double d2, const double* rv) {
+            const V wa = static_cast<const V*>(s.wA)[j];
+            const V wb = static_cast<const V*>(s.wB)[j];
+            T n[3], vw[3], gap_j, Tw, kw;
+            n[0] = wa.x;
+            n[1] = wa.y;
+            vw[0] = wb.x;
+            vw[1] = wb.y;
+            if (D == 3) {
+                n[2] = wa.z;
+                gap_j = wa.w;
+                vw[2] = wb.z;
+                Tw = wb.w;
+                kw = THERMAL ? ld<T>(s.wkap, j) : T(0);
+            } else {
+                gap_j = wa.z;
+                Tw = wa.w;
+                kw = wb.z;
+            }
+            const T r = psqrt((T)d2);
+            T W, dW;
+            KQ::wdw(r * inv_h, kA, kB, W, dW);
+            const T sg = d2 == 0.0 ? T(0) : dW * prcp(r);
+            T vv[3];
+            T vn = T(0), rn = T(0);
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                vv[k] = me.v[k] - vw[k];
+                vn += vv[k] * n[k];
+                rn += (T)rv[k] * n[k];
+            }
+            // wallForce! (Rogers & Dalrymple 2008)
+            {
+                const T psi = fabs(rn);
+                const T a = r * r - psi * psi;
+                const T xi_ = psqrt(a > T(0) ? a : T(0));
+                const T q = psi / gap_j;
+                if (!(q > T(1) || xi_ > gap_j)) {
+                    const T p_xi = fabs((T(1) + cos(T(3.14159265358979323846) * xi_ / gap_j)) * T(0.5));
+                    const T beta = vn > T(0) ? T(0) : T(1);
+                    const T wf = (T(0.01) * (c_i * c_i) + beta * c_i * fabs(vn)) * inv_h * fabs(T(1) - q) *
+                                 prsqrt(q) * p_xi;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) dv[k] += wf * n[k];
+                }
+            }
+            const T r2e = r * r + eta2;
+            const T vf = T(2) * nu0 * r * dW * inv_rho_i * prcp(r2e);
+            T ps = T(0);
+            if (wall_pressure) {
+                T P = pr2_i + P0;
+                P += fabs(P) * T(0.01) * W * inv_Wgap_i;
+                ps = -P * sg;
+            }
+#pragma unroll
+            for (int k = 0; k < D; ++k) dv[k] += m_i * (ps * (T)rv[k] + vf * (vv[k] - vn * n[k]));
+            if (THERMAL && s.wkind[j] == ESPH_WALL_THERMOSTATIC)
+                dTw += (kap_i + kw) * r * dW * (T_i - Tw) * prcp(r2e);
+        });
+        if (THERMAL) dT += dTw * th_w;
+    }
+    {
+#pragma clang fp contract(off)
+        const double dt = s.dt;
+        double bf[3];
+        if (THERMAL) {
+            const double Tn = S.T[i] + dt * (double)dT;
+            s.Tn[i] = Tn;
+#pragma unroll
+            for (int k = 0; k < D; ++k) bf[k] = -s.g[k] * s.beta * (Tn - s.t0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; ++k) bf[k] = s.g[k];
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            double v = S.v[i * D + k];
+            v += bf[k] * dt;
+            const double dvk = (double)dv[k];
+            v += dvk * dt;
+            s.vn[i * D + k] = v;
+            s.xn[i * D + k] = xi[k] + (v * dt - dvk * (dt * dt) / 2.0);
+        }
+    }
+}
+
+template <int D, int K, typename T, bool THERMAL, bool SCAN>
 __global__ __launch_bounds__(kBlock) void k_forces(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, int ntiles) {
-    using SK = SmoothKernelFn<T, K>;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
-        const int i = tile.begin + (int)threadIdx.x;
-        if (i >= dyn.n_live) continue;
-        const FluidSet& S = s.set[dyn.parity];
-        const bool thermal = (s.flags & FLAG_THERMAL) != 0;
-        const T h = (T)s.h, sigma = (T)s.sigma;
-        const T eta2 = (T)(0.01 * (s.h * s.h));
-        const T nu0 = (T)s.nu0;
-        double xi[3];
-        load_x<D>(S.x, i, xi);
-        T vi[3];
-#pragma unroll
-        for (int k = 0; k < D; ++k) vi[k] = ld<T>(S.vs, i * D + k);
-        const T rho_i = sizeof(T) == sizeof(double) ? (T)S.rho[i] : ld<T>(S.rhos, i);
-        const T pr2_i = ld<T>(S.pr2s, i);
-        const T gap_i = ld<T>(S.gaps, i);
-        const T m_i = ld<T>(S.ms, i);
-        const T mu_i = nu0 * rho_i;
-        T T_i = T(0), kap_i = T(0), cp_i = T(1);
-        if (thermal) {
-            T_i = ld<T>(S.Ts, i);
-            kap_i = ld<T>(S.kappas, i);
-            cp_i = ld<T>(S.cps, i);
-        }
-        int c[3];
-        cell_of<D>(s, xi, c);
-        T dv[3] = {T(0), T(0), T(0)};
-        T dT = T(0);
-        for_each_neighbour<D>(s, s.cell_start, S.x, xi, c, i, [&](int j, double d2, const double* rv) {
-            const double rd = sqrt(d2);
-            const T r = (T)rd;
-            const T W = SK::value(r, h, sigma);
-            const T dW = SK::gradient(r, h, sigma);
-            const T inv = T(1) / r;
-            T g[3];
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                g[k] = dW * ((T)rv[k] * inv);
-                if (rd == 0.0) g[k] = T(0);
-            }
-            const T m_j = ld<T>(S.ms, j);
-            const T rho_j = ld<T>(S.rhos, j);
-            // pressure
-            T P = pr2_i + ld<T>(S.pr2s, j);
-            P += fabs(P) * T(0.01) * W / SK::value((gap_i + ld<T>(S.gaps, j)) / T(2), h, sigma);
-            // viscosity
-            const T sum_rho = rho_i + rho_j;
-            const T mu_j = nu0 * rho_j;
-            const T vf = T(4) * (mu_i + mu_j) * r * dW / (sum_rho * sum_rho) / (r * r + eta2);
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                dv[k] += -m_j * P * g[k];
-                dv[k] += m_j * vf * (vi[k] - ld<T>(S.vs, j * D + k));
-            }
-            if (thermal) {
-                const T kap_j = ld<T>(S.kappas, j);
-                const T keq = T(2) * kap_i * kap_j / (kap_i + kap_j);
-                const T dq = T(2) * keq * r * dW / rho_i / rho_j / (r * r + eta2) * (T_i - ld<T>(S.Ts, j));
-                dT += dq * m_j / cp_i;
-            }
+        if (((s.flags & FLAG_THERMAL) != 0) != THERMAL) continue;
+        for_work<SCAN>(s, dyn, tile, [&](int i) {
+            if (!SCAN && overflowed(s, i)) return;
+            forces_body<D, K, T, THERMAL, SCAN>(s, s.set[dyn.parity], i);
         });
-        if (s.nw > 0) {
-            const bool wall_pressure = (s.flags & FLAG_WALL_PRESSURE) != 0;
-            const T c_i = (T)S.c[i];
-            const T P0 = (T)(s.p0 / (s.rho0 * s.rho0));
-            const T Wgap_i = SK::value(gap_i, h, sigma);
-            const T rho0 = (T)s.rho0;
-            const T inv_cpm = (T)(1.0 / s.cp);
-            for_each_neighbour<D>(s, s.wcell_start, s.wx, xi, c, -1, [&](int j, double d2, const double* rv) {
-                const double rd = sqrt(d2);
-                const T r = (T)rd;
-                const T W = SK::value(r, h, sigma);
-                const T dW = SK::gradient(r, h, sigma);
-                const T inv = T(1) / r;
-                T g[3], n[3], vv[3];
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    g[k] = dW * ((T)rv[k] * inv);
-                    if (rd == 0.0) g[k] = T(0);
-                    n[k] = ld<T>(s.wns, j * D + k);
-                    vv[k] = vi[k] - ld<T>(s.wvs, j * D + k);
-                }
-                const T gap_j = ld<T>(s.wgaps, j);
-                T vn = T(0), rn = T(0);
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    vn = vn + vv[k] * n[k];
-                    rn = rn + (T)rv[k] * n[k];
-                }
-                // wallForce! (Rogers & Dalrymple 2008)
-                {
-                    const T psi = fabs(rn);
-                    const T a = r * r - psi * psi;
-                    const T xi_ = sqrt(a > T(0) ? a : T(0));
-                    const T q = psi / gap_j;
-                    if (!(q > T(1) || xi_ > gap_j)) {
-                        const T p_xi = fabs((T(1) + cos(T(3.14159265358979323846) * xi_ / gap_j)) / T(2));
-                        const T beta = vn > T(0) ? T(0) : T(1);
-                        const T r_psi = (T(0.01) * (c_i * c_i) + beta * c_i * fabs(vn)) / h * fabs(T(1) - q) / sqrt(q);
-                        const T wf = r_psi * p_xi;
-#pragma unroll
-                        for (int k = 0; k < D; ++k) dv[k] += wf * n[k];
-                    }
-                }
-                if (wall_pressure) {
-                    T P = pr2_i + P0;
-                    P += fabs(P) * T(0.01) * W / Wgap_i;
-#pragma unroll
-                    for (int k = 0; k < D; ++k) dv[k] += -m_i * P * g[k];
-                }
-                {
-                    const T sum_rho = T(2) * rho_i;
-                    const T vf = T(8) * mu_i * r * dW / (sum_rho * sum_rho) / (r * r + eta2);
-#pragma unroll
-                    for (int k = 0; k < D; ++k) dv[k] += m_i * vf * (vv[k] - vn * n[k]);
-                }
-                if (thermal && s.wkind[j] == ESPH_WALL_THERMOSTATIC) {
-                    const T dq = inv_cpm * (kap_i + ld<T>(s.wkappas, j)) * r * dW / rho_i / rho0 / (r * r + eta2) *
-                                 (T_i - ld<T>(s.wTs, j));
-                    dT += dq * m_i;
-                }
-            });
-        }
-        {
-#pragma clang fp contract(off)
-            const double dt = s.dt;
-            double bf[3];
-            if (thermal) {
-                const double Tn = S.T[i] + dt * (double)dT;
-                s.Tn[i] = Tn;
-#pragma unroll
-                for (int k = 0; k < D; ++k) bf[k] = -s.g[k] * s.beta * (Tn - s.t0);
-            } else {
-#pragma unroll
-                for (int k = 0; k < D; ++k) bf[k] = s.g[k];
-            }
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                double v = S.v[i * D + k];
-                v += bf[k] * dt;
-                const double dvk = (double)dv[k];
-                v += dvk * dt;
-                s.vn[i * D + k] = v;
-                s.xn[i * D + k] = xi[k] + (v * dt - dvk * (dt * dt) / 2.0);
-            }
-        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // reconstructScalar!(fluid, :rho_, FF pairs, kernel) (GroupAction/
 // ReconstructValue.jl:22-57): Shepard filter over the step-start pairs with
-// the post-continuity density, summed in binary64 in every form.
+// the post-continuity density, in binary64 in every form.
 // ---------------------------------------------------------------------------
-template <int D, int K>
+template <int D, int K, bool SCAN>
 __global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__ scenes,
                                                     const Tile* __restrict__ tiles, int ntiles) {
-    using SK = SmoothKernelFn<double, K>;
+    using KQ = KernelQ<double, K>;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
         if (s.kreinit <= 0 || (dyn.step + 1) % s.kreinit != 0) continue;
-        const int i = tile.begin + (int)threadIdx.x;
-        if (i >= dyn.n_live) continue;
+        for_work<SCAN>(s, dyn, tile, [&](int i) {
+        if (!SCAN && overflowed(s, i)) return;
         const FluidSet& S = s.set[dyn.parity];
         double xi[3];
         load_x<D>(S.x, i, xi);
-        int c[3];
-        cell_of<D>(s, xi, c);
         double sw = 0.0, ss = 0.0;
-        for_each_neighbour<D>(s, s.cell_start, S.x, xi, c, i, [&](int j, double d2, const double*) {
-            const double W = SK::value(sqrt(d2), s.h, s.sigma);
+        for_pairs<D, SCAN>(s, s.cell_start, S.x, xi, i, i, s.nbr, s.nnb[i], [&](int j, double d2, const double*) {
+            double W, dW;
+            KQ::wdw(sqrt(d2) * s.inv_h, s.kA, s.kB, W, dW);
             const double rho_j = S.rho[j];
             const double a = W * S.m[j] / rho_j;
             sw += a;
@@ -321,6 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__
             ss += a * rho_i;
             s.rhon[i] = ss / sw;
         }
+        });
     }
 }
 
@@ -349,6 +510,31 @@ __global__ __launch_bounds__(kBlock) void k_sort_clear(const SceneDev* __restric
             if (c <= s.ncell) s.count[c] = 0;
         }
     }
+}
+
+// Slot of this lane's key in count[key]: lanes of one wavefront holding the
+// same key in consecutive lanes (the usual case: particles are sorted by
+// cell) share one atomic; the run head adds the run length and each lane
+// takes head result + its offset in the run.  Works for any active mask.
+__device__ __forceinline__ int wave_counted_slot(int32_t* count, int key) {
+    const unsigned lane = threadIdx.x & 63u;
+    const uint64_t active = __ballot(1);
+    const int prev = __shfl(key, (int)((lane + 63u) & 63u), 64);
+    // run head: first active lane, or the previous lane inactive, or a different key
+    const bool prev_active = lane > 0 && ((active >> (lane - 1)) & 1ull);
+    const bool head = !prev_active || prev != key;
+    const uint64_t heads = __ballot(head) & active;
+    // this lane's run head and the next head after it
+    const uint64_t upto = heads & ((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
+    const int my_head = 63 - __clzll((long long)upto);
+    // run end: the first lane after this one that is a head or inactive
+    const uint64_t above = (lane == 63u) ? 0ull : ~((2ull << lane) - 1ull);
+    const uint64_t stop = (heads | ~active) & above;
+    const int run_end = stop ? __ffsll((long long)stop) - 1 : 64;
+    int base = 0;
+    if (head) base = atomicAdd(&count[key], run_end - (int)lane);
+    base = __shfl(base, my_head, 64);
+    return base + ((int)lane - my_head);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict__ scenes,
@@ -383,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
         }
         if (!finite) atomicOr(&s.dyn->err, 1);
         s.key[i] = key;
-        s.slot[i] = atomicAdd(&s.count[key], 1);
+        s.slot[i] = wave_counted_slot(s.count, key);
     }
 }
 
@@ -507,39 +693,33 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
         const FluidSet& S = s.set[dyn.parity];
         const FluidSet& O = s.set[dyn.parity ^ 1];
         const int D = s.dim;
-        const bool f32 = s.prec == ESPH_FP32_BULK;
+        const bool thermal = (s.flags & FLAG_THERMAL) != 0;
+        double v[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < D; ++k) {
-            const double x = s.xn[i * D + k];
-            const double v = s.vn[i * D + k];
-            O.x[p * D + k] = x;
-            O.v[p * D + k] = v;
-            if (f32) static_cast<float*>(O.vs)[p * D + k] = (float)v;
+            v[k] = s.vn[i * D + k];
+            O.x[p * D + k] = s.xn[i * D + k];
+            O.v[p * D + k] = v[k];
         }
         const bool rho_new = mode == 1 || shepard_due(s, dyn);
-        const double rho = rho_new ? s.rhon[i] : S.rho[i];
-        O.rho[p] = rho;
+        O.rho[p] = rho_new ? s.rhon[i] : S.rho[i];
         O.p[p] = S.p[i];
         O.c[p] = S.c[i];
         const double m = S.m[i], gap = S.gap[i];
         O.m[p] = m;
         O.gap[p] = gap;
         O.id[p] = S.id[i];
-        if (f32) {
-            static_cast<float*>(O.ms)[p] = (float)m;
-            static_cast<float*>(O.gaps)[p] = (float)gap;
-            static_cast<float*>(O.rhos)[p] = (float)rho;
-        }
-        if (s.flags & FLAG_THERMAL) {
-            const double T = s.Tn[i], kap = S.kappa[i], cp = S.cp[i];
-            O.T[p] = T;
+        double Tt = 0.0, kap = 0.0;
+        if (thermal) {
+            Tt = s.Tn[i];
+            kap = S.kappa[i];
+            O.T[p] = Tt;
             O.kappa[p] = kap;
-            O.cp[p] = cp;
-            if (f32) {
-                static_cast<float*>(O.Ts)[p] = (float)T;
-                static_cast<float*>(O.kappas)[p] = (float)kap;
-                static_cast<float*>(O.cps)[p] = (float)cp;
-            }
+            O.cp[p] = S.cp[i];
         }
+        if (s.prec == ESPH_FP32_BULK)
+            store_sorted_rec<float>(O, D, p, v, m, gap, Tt, kap, thermal);
+        else
+            store_sorted_rec<double>(O, D, p, v, m, gap, Tt, kap, thermal);
     }
 }
 
@@ -550,6 +730,7 @@ __global__ void k_sort_finish(const SceneDev* __restrict__ scenes, int nscenes, 
     SceneDyn* d = s.dyn;
     d->n_live = s.cell_start[s.ncell];
     d->parity ^= 1;
+    d->novf = 0;
     if (mode == 0) d->step += 1;
 }
 
@@ -620,38 +801,65 @@ __global__ void k_kernel_eval(int64_t n, const double* __restrict__ r, double* _
 
 static inline int grid_for(int ntiles) { return ntiles > 0 ? ntiles : 1; }
 
-hipError_t launch_continuity(const Bucket& b, hipStream_t st) {
+template <typename L>
+static hipError_t launch_both(const Bucket& b, hipStream_t st, L&& launch) {
     if (b.ntiles <= 0) return hipSuccess;
-    dim3 g(grid_for(b.ntiles)), blk(kBlock);
-    ESPH_DISPATCH_K(b.kernel, {
-        if (b.dim == 2 && b.prec == ESPH_FP64) hipLaunchKernelGGL((k_continuity<2, K, double>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
-        else if (b.dim == 2) hipLaunchKernelGGL((k_continuity<2, K, float>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
-        else if (b.prec == ESPH_FP64) hipLaunchKernelGGL((k_continuity<3, K, double>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
-        else hipLaunchKernelGGL((k_continuity<3, K, float>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
-    });
+    launch(std::false_type{}, dim3(grid_for(b.ntiles)), b.tiles, b.ntiles);
+    launch(std::true_type{}, dim3(kOvfTiles * b.nscenes), b.ovf_tiles, kOvfTiles * b.nscenes);
     return hipGetLastError();
+}
+
+hipError_t launch_continuity(const Bucket& b, hipStream_t st) {
+    dim3 blk(kBlock);
+    hipError_t e = hipSuccess;
+    ESPH_DISPATCH_K(b.kernel, {
+        e = launch_both(b, st, [&](auto scan, dim3 g, const Tile* tl, int nt) {
+            constexpr bool S = decltype(scan)::value;
+            if (b.dim == 2 && b.prec == ESPH_FP64) hipLaunchKernelGGL((k_continuity<2, K, double, S>), g, blk, 0, st, b.scenes, tl, nt);
+            else if (b.dim == 2) hipLaunchKernelGGL((k_continuity<2, K, float, S>), g, blk, 0, st, b.scenes, tl, nt);
+            else if (b.prec == ESPH_FP64) hipLaunchKernelGGL((k_continuity<3, K, double, S>), g, blk, 0, st, b.scenes, tl, nt);
+            else hipLaunchKernelGGL((k_continuity<3, K, float, S>), g, blk, 0, st, b.scenes, tl, nt);
+        });
+    });
+    return e;
+}
+
+template <bool TH>
+static hipError_t launch_forces_t(const Bucket& b, hipStream_t st) {
+    dim3 blk(kBlock);
+    hipError_t e = hipSuccess;
+    ESPH_DISPATCH_K(b.kernel, {
+        e = launch_both(b, st, [&](auto scan, dim3 g, const Tile* tl, int nt) {
+            constexpr bool S = decltype(scan)::value;
+            if (b.dim == 2 && b.prec == ESPH_FP64) hipLaunchKernelGGL((k_forces<2, K, double, TH, S>), g, blk, 0, st, b.scenes, tl, nt);
+            else if (b.dim == 2) hipLaunchKernelGGL((k_forces<2, K, float, TH, S>), g, blk, 0, st, b.scenes, tl, nt);
+            else if (b.prec == ESPH_FP64) hipLaunchKernelGGL((k_forces<3, K, double, TH, S>), g, blk, 0, st, b.scenes, tl, nt);
+            else hipLaunchKernelGGL((k_forces<3, K, float, TH, S>), g, blk, 0, st, b.scenes, tl, nt);
+        });
+    });
+    return e;
 }
 
 hipError_t launch_forces(const Bucket& b, hipStream_t st) {
-    if (b.ntiles <= 0) return hipSuccess;
-    dim3 g(grid_for(b.ntiles)), blk(kBlock);
-    ESPH_DISPATCH_K(b.kernel, {
-        if (b.dim == 2 && b.prec == ESPH_FP64) hipLaunchKernelGGL((k_forces<2, K, double>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
-        else if (b.dim == 2) hipLaunchKernelGGL((k_forces<2, K, float>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
-        else if (b.prec == ESPH_FP64) hipLaunchKernelGGL((k_forces<3, K, double>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
-        else hipLaunchKernelGGL((k_forces<3, K, float>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
-    });
-    return hipGetLastError();
+    if (b.has_thermal) {
+        hipError_t e = launch_forces_t<true>(b, st);
+        if (e != hipSuccess) return e;
+    }
+    if (b.has_plain) return launch_forces_t<false>(b, st);
+    return hipSuccess;
 }
 
 hipError_t launch_shepard(const Bucket& b, hipStream_t st) {
-    if (b.ntiles <= 0) return hipSuccess;
-    dim3 g(grid_for(b.ntiles)), blk(kBlock);
+    dim3 blk(kBlock);
+    hipError_t e = hipSuccess;
     ESPH_DISPATCH_K(b.kernel, {
-        if (b.dim == 2) hipLaunchKernelGGL((k_shepard<2, K>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
-        else hipLaunchKernelGGL((k_shepard<3, K>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles);
+        e = launch_both(b, st, [&](auto scan, dim3 g, const Tile* tl, int nt) {
+            constexpr bool S = decltype(scan)::value;
+            if (b.dim == 2) hipLaunchKernelGGL((k_shepard<2, K, S>), g, blk, 0, st, b.scenes, tl, nt);
+            else hipLaunchKernelGGL((k_shepard<3, K, S>), g, blk, 0, st, b.scenes, tl, nt);
+        });
     });
-    return hipGetLastError();
+    return e;
 }
 
 hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st) {

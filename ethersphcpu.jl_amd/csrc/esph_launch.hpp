@@ -16,6 +16,10 @@ struct Bucket {
     const SceneDev* scenes;  // device table
     const Tile* tiles;       // device: particle tiles of the bucket's scenes
     int ntiles;
+    const Tile* ovf_tiles;   // device: kOvfTiles tiles per scene over SceneDev::ovf
+    int nscenes;
+    bool has_thermal;        // some scene of the bucket is thermal
+    bool has_plain;          // some scene of the bucket is not
 };
 
 // Tables for the dimension-agnostic sort kernels of a whole batch.

@@ -8,6 +8,9 @@
 
 #include "../../include/ethersph.h"
 #include "esph_device.hpp"
+#include "esph_launch.hpp"
+
+struct esph_scene;
 
 struct esph_ctx {
     int device = 0;
@@ -37,12 +40,30 @@ struct WallHost {
 struct WallStage {
     std::vector<double> x, n_, v, gap, T, kappa;
     std::vector<int32_t> kind, wid, wcs;
-    std::vector<float> fn, fv, fgap, fT, fkap;
+    std::vector<float> fA, fB, fK;
+    std::vector<double> dA, dB, dK;
 };
 
 struct TimedStep {
     hipEvent_t ev[5];
     bool shepard;
+};
+
+// Launch plan of one step for a set of scenes (a batch, or one scene in
+// graph mode): device scene table, per-bucket tile tables, sort plan and the
+// optionally captured step graph.
+struct StepPlan {
+    Arena arena;
+    std::vector<Bucket> buckets;
+    SortPlan plan{};
+    int nmembers = 0;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int build(esph_ctx* ctx, const std::vector<esph_scene*>& members);
+    hipError_t launch_step(hipStream_t st) const;
+    int step(esph_ctx* ctx, int64_t nsteps);
+    void release();
 };
 
 int fail(esph_ctx* ctx, int code, const std::string& msg);
@@ -61,10 +82,11 @@ struct esph_scene {
     int nptiles = 0;
     esph::Tile* ctiles = nullptr;
     int nctiles = 0;
+    esph::Tile* otiles = nullptr;  // overflow-list tiles of the SCAN kernels
     esph::Arena arena;
     esph::WallHost walls;
     esph::WallStage wall_stage;
-    std::vector<esph::Tile> tile_stage_p, tile_stage_c;
+    std::vector<esph::Tile> tile_stage_p, tile_stage_c, tile_stage_o;
     bool walls_set = false;
     bool allocated = false;
     bool fluid_ready = false;
@@ -74,6 +96,8 @@ struct esph_scene {
     std::vector<hipEvent_t> timing_events;
     size_t ev_next = 0;
     std::vector<esph::TimedStep> timed;
+    bool use_graph = false;     // step through a captured one-step graph
+    esph::StepPlan gplan;       // this scene alone (graph mode)
 };
 
 #define HIPCHK(ctx, expr)                                          \

@@ -11,7 +11,7 @@ from .models import (CommonWeaklyCompressibleLiquidModel, ThermalWeaklyCompressi
 from .neighbour import neighborlist
 from .particles import (CommonLiquidParticle, CompulsiveWallParticle, LiquidParticles, StaticVelocityWallParticle,
                         ThermalLiquidParticle, ThermostaticWallParticle, WallParticles)
-from .solve import AABBCheck, Scene, solve, solve_
+from .solve import AABBCheck, Batch, Scene, solve, solve_
 from .sph_kernel import (CubicSpline, Gaussian, SmoothKernel, WendlandC2, WendlandC4, kernelGradient,
                          kernelValue)
 from .time_discretization import (DensityReinitializedFixedTimeIntervalForwardEuler, isCheckStep,

@@ -29,6 +29,7 @@ WALL_COMPULSIVE, WALL_STATIC_VELOCITY, WALL_THERMOSTATIC = 0, 1, 2
 FIELD_X, FIELD_V, FIELD_RHO, FIELD_P, FIELD_C, FIELD_MASS, FIELD_GAP, FIELD_T, FIELD_KAPPA, \
     FIELD_CP, FIELD_ID = range(11)
 PAIRS_FLUID_FLUID, PAIRS_FLUID_WALL = 0, 1
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 D3 = c_double * 3
 
@@ -98,6 +99,22 @@ def lib() -> ctypes.CDLL:
         "esph_scene_neighbours": ([vp, c_int32, POINTER(c_int32), c_int64, POINTER(c_int64)], c_int32),
         "esph_scene_enable_timing": ([vp, c_int32], c_int32),
         "esph_scene_phase_times": ([vp, DP, POINTER(c_int64)], c_int32),
+        "esph_scene_stats": ([vp, POINTER(c_int64)], c_int32),
+        "esph_scene_enable_graph": ([vp, c_int32], c_int32),
+        "esph_batch_create": ([vp, c_int32, POINTER(vp), POINTER(vp)], c_int32),
+        "esph_batch_destroy": ([vp], c_int32),
+        "esph_batch_enable_graph": ([vp, c_int32], c_int32),
+        "esph_batch_step": ([vp, c_int64], c_int32),
+        "esph_batch_sync": ([vp], c_int32),
+        "esph_server_create": ([vp, c_int32, POINTER(vp)], c_int32),
+        "esph_server_destroy": ([vp], c_int32),
+        "esph_server_submit": ([vp, POINTER(SceneDesc), c_int32, POINTER(WallArrays), POINTER(FluidArrays), c_int64,
+                                POINTER(c_int64)], c_int32),
+        "esph_server_advance": ([vp, c_int64, POINTER(c_int64)], c_int32),
+        "esph_server_status": ([vp, c_int64, POINTER(c_int32), POINTER(c_int64)], c_int32),
+        "esph_server_result": ([vp, c_int64, c_int32, DP, c_int64, POINTER(c_int64)], c_int32),
+        "esph_server_release": ([vp, c_int64], c_int32),
+        "esph_server_stats": ([vp, DP], c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

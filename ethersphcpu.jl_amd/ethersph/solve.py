@@ -226,6 +226,14 @@ class Scene:
         self.ctx.check(self.L.esph_scene_phase_times(self.handle, ms, byref(n)))
         return {"sort": ms[0], "continuity": ms[1], "forces": ms[2], "shepard": ms[3], "steps": n.value}
 
+    def stats(self) -> dict:
+        v = (c_int64 * 4)()
+        self.ctx.check(self.L.esph_scene_stats(self.handle, v))
+        return {"list_overflow": v[0], "list_width_ff": v[1], "list_width_fw": v[2], "cells": v[3]}
+
+    def enable_graph(self, on: bool = True) -> None:
+        self.ctx.check(self.L.esph_scene_enable_graph(self.handle, int(on)))
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.L.esph_scene_destroy(self.handle)
@@ -300,3 +308,38 @@ def solve(*args, check=None, precision: str = "FP64", ctx=None, on_output=None):
 # Python names cannot end in "!"; these aliases keep the reference's spelling
 # recognisable.
 solve_ = solve
+
+
+class Batch:
+    """Several resident scenes stepped together (esph_batch): one launch per
+    (dim, kernel, precision) class and phase for all of them."""
+
+    def __init__(self, scenes: list, graph: bool = True):
+        if not scenes:
+            raise ValueError("empty batch")
+        self.scenes = list(scenes)
+        self.ctx = self.scenes[0].ctx
+        self.L = self.ctx._L
+        arr = (c_void_p * len(self.scenes))(*[s.handle.value for s in self.scenes])
+        h = c_void_p()
+        self.ctx.check(self.L.esph_batch_create(self.ctx.handle, len(self.scenes), arr, byref(h)))
+        self.handle = h
+        if graph:
+            self.ctx.check(self.L.esph_batch_enable_graph(self.handle, 1))
+
+    def step(self, nsteps: int = 1) -> None:
+        self.ctx.check(self.L.esph_batch_step(self.handle, nsteps))
+
+    def sync(self) -> None:
+        self.ctx.check(self.L.esph_batch_sync(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.L.esph_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

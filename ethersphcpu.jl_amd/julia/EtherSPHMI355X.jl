@@ -1,0 +1,262 @@
+#=
+  EtherSPHMI355X.jl — Julia front end of libethersph_hip.so.
+
+  Adds GPU methods of EtherSPHCPU.solve! selected by a trailing `MI355X()`
+  argument; the particle types, kernels, models and time discretisation stay
+  the reference's own.  Particle vectors are packed into contiguous
+  Matrix{Float64}(dim, n) arrays (column-major == particle-major, the layout
+  the C ABI expects), handed over once, stepped on the GPU and unpacked.
+
+  Interfaces replaced (EtherSPHCPU.jl):
+    solve!(fluid, wall, kernel, model, td, vtp_io; check)            Solve/FluidWallSolve.jl:101
+    solve!(fluid, static_velocity_wall, ...)                          Solve/FluidVelocitySolve.jl:104
+    solve!(fluid, thermostatic_wall, ..., ::ThermalWeakly...Model)    Solve/FluidThermostaticWallSolve.jl:127
+    solve!(fluid, wall, thermostatic_wall, ..., ::ThermalWeakly...)   Solve/FluidCompulsiveThermostaticWallSolve.jl:154
+    kernelValue / kernelGradient                                      SPHKernel/SmoothKernel/*.jl
+    CellListMap.neighborlist (as used by findNeighbours!)             Neighbour/Neighbour.jl:88
+  `check` closures cannot run on the device; pass an AABBCheck instead.
+=#
+module EtherSPHMI355X
+
+using EtherSPHCPU
+
+export MI355X, AABBCheck, GPUContext, gpu_neighborlist
+
+const libesph = joinpath(@__DIR__, "..", "libethersph_hip.so")
+
+struct MI355X
+    device::Int32
+end
+MI355X() = MI355X(0)
+
+"The `check` predicate as data: keep iff lo <= x <= hi on every axis."
+struct AABBCheck
+    lo::Vector{Float64}
+    hi::Vector{Float64}
+end
+
+# ---- C structures (include/ethersph.h) --------------------------------------
+struct EsphSceneDesc
+    dim::Int32
+    kernel::Int32
+    influence_radius::Float64
+    solver::Int32
+    precision::Int32
+    rho_0::Float64
+    c_0::Float64
+    p_0::Float64
+    gamma::Int32
+    mu_0::Float64
+    body_force::NTuple{3,Float64}
+    t_0::Float64
+    kappa_0::Float64
+    cp::Float64
+    beta::Float64
+    dt::Float64
+    density_reinitialized_interval::Int64
+    check_interval::Int64
+    check_enabled::Int32
+    check_min::NTuple{3,Float64}
+    check_max::NTuple{3,Float64}
+    fluid_capacity::Int64
+    domain_set::Int32
+    domain_min::NTuple{3,Float64}
+    domain_max::NTuple{3,Float64}
+end
+
+struct EsphFluidArrays
+    n::Int64
+    x::Ptr{Float64}
+    v::Ptr{Float64}
+    rho::Ptr{Float64}
+    p::Ptr{Float64}
+    c::Ptr{Float64}
+    mass::Ptr{Float64}
+    gap::Ptr{Float64}
+    t::Ptr{Float64}
+    kappa::Ptr{Float64}
+    cp::Ptr{Float64}
+end
+
+struct EsphWallArrays
+    kind::Int32
+    n::Int64
+    x::Ptr{Float64}
+    normal::Ptr{Float64}
+    gap::Ptr{Float64}
+    v::Ptr{Float64}
+    t::Ptr{Float64}
+    kappa::Ptr{Float64}
+end
+
+kernel_kind(::CubicSpline) = Int32(0)
+kernel_kind(::Gaussian) = Int32(1)
+kernel_kind(::WendlandC2) = Int32(2)
+kernel_kind(::WendlandC4) = Int32(3)
+
+wall_kind(::AbstractVector{<:CompulsiveWallParticle}) = Int32(0)
+wall_kind(::AbstractVector{<:StaticVelocityWallParticle}) = Int32(1)
+wall_kind(::AbstractVector{<:ThermostaticWallParticle}) = Int32(2)
+
+# ---- context ---------------------------------------------------------------------
+mutable struct GPUContext
+    handle::Ptr{Cvoid}
+    function GPUContext(device::Integer = 0, hbm_budget::Integer = 0)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:esph_ctx_create, libesph), Cint, (Int32, UInt64, Ptr{Ptr{Cvoid}}), device, hbm_budget, h)
+        rc == 0 || error("esph_ctx_create failed ($rc)")
+        ctx = new(h[])
+        finalizer(c -> ccall((:esph_ctx_destroy, libesph), Cint, (Ptr{Cvoid},), c.handle), ctx)
+        return ctx
+    end
+end
+
+function chk(ctx::GPUContext, rc::Cint)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:esph_last_error, libesph), Cstring, (Ptr{Cvoid},), ctx.handle))
+    error("ethersph error $rc: $msg")
+end
+
+const CONTEXTS = Dict{Int32,GPUContext}()
+context(dev::MI355X) = get!(() -> GPUContext(dev.device), CONTEXTS, dev.device)
+
+# ---- packing ------------------------------------------------------------------------
+packvec(ps, f) = reduce(hcat, [getfield(p, f) for p in ps]; init = zeros(Float64, length(getfield(first(ps), f)), 0))
+packsc(ps, f) = Float64[getfield(p, f) for p in ps]
+
+function descriptor(dim, kernel, model, td, solver, check)
+    bf = ntuple(k -> k <= dim ? Float64(model.body_force_vec_[k]) : 0.0, 3)
+    thermal = model isa ThermalWeaklyCompressibleLiquidModel
+    lo = ntuple(k -> (check !== nothing && k <= dim) ? check.lo[k] : -Inf, 3)
+    hi = ntuple(k -> (check !== nothing && k <= dim) ? check.hi[k] : Inf, 3)
+    return EsphSceneDesc(dim, kernel_kind(kernel), kernel.influence_radius_, solver, 0,
+                         model.rho_0_, model.c_0_, model.p_0_, model.gamma_, model.mu_0_, bf,
+                         thermal ? model.t_0_ : 0.0, thermal ? model.kappa_0_ : 0.0,
+                         thermal ? model.cp_ : 0.0, thermal ? model.beta_ : 0.0,
+                         td.dt_, td.density_reinitialized_interval_, td.check_interval_,
+                         check === nothing ? 0 : 1, lo, hi, 0, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+end
+
+function run_gpu!(fluid, walls, kernel, model, td, solver, check, dev::MI355X)
+    ctx = context(dev)
+    dim = length(first(fluid).x_vec_)
+    desc = Ref(descriptor(dim, kernel, model, td, solver, check))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    chk(ctx, ccall((:esph_scene_create, libesph), Cint, (Ptr{Cvoid}, Ptr{EsphSceneDesc}, Ptr{Ptr{Cvoid}}),
+                     ctx.handle, desc, h))
+    scene = h[]
+    thermal = first(fluid) isa ThermalLiquidParticle && model isa ThermalWeaklyCompressibleLiquidModel
+    try
+        # walls: all groups at once
+        wpacked = [(wall_kind(w), length(w), packvec(w, :x_vec_), packvec(w, :normal_vec_), packsc(w, :gap_),
+                    w isa AbstractVector{<:StaticVelocityWallParticle} ? packvec(w, :v_vec_) : zeros(dim, length(w)),
+                    w isa AbstractVector{<:ThermostaticWallParticle} ? packsc(w, :t_) : zeros(length(w)),
+                    w isa AbstractVector{<:ThermostaticWallParticle} ? packsc(w, :kappa_) : zeros(length(w)))
+                   for w in walls]
+        GC.@preserve wpacked begin
+            wa = [EsphWallArrays(k, n, pointer(x), pointer(nn), pointer(g), pointer(v), pointer(t), pointer(kap))
+                  for (k, n, x, nn, g, v, t, kap) in wpacked]
+            chk(ctx, ccall((:esph_scene_set_walls, libesph), Cint, (Ptr{Cvoid}, Int32, Ptr{EsphWallArrays}),
+                             scene, length(wa), wa))
+        end
+        x = packvec(fluid, :x_vec_); v = packvec(fluid, :v_vec_)
+        rho = packsc(fluid, :rho_); p = packsc(fluid, :p_); c = packsc(fluid, :c_)
+        m = packsc(fluid, :mass_); gap = packsc(fluid, :gap_)
+        t = thermal ? packsc(fluid, :t_) : Float64[]
+        kap = thermal ? packsc(fluid, :kappa_) : Float64[]
+        cp = thermal ? packsc(fluid, :cp_) : Float64[]
+        GC.@preserve x v rho p c m gap t kap cp begin
+            fa = Ref(EsphFluidArrays(length(fluid), pointer(x), pointer(v), pointer(rho), pointer(p), pointer(c),
+                                     pointer(m), pointer(gap), thermal ? pointer(t) : C_NULL,
+                                     thermal ? pointer(kap) : C_NULL, thermal ? pointer(cp) : C_NULL))
+            chk(ctx, ccall((:esph_scene_set_fluid, libesph), Cint, (Ptr{Cvoid}, Ptr{EsphFluidArrays}), scene, fa))
+        end
+        chk(ctx, ccall((:esph_scene_step, libesph), Cint, (Ptr{Cvoid}, Int64), scene, td.total_step_))
+        chk(ctx, ccall((:esph_scene_sync, libesph), Cint, (Ptr{Cvoid},), scene))
+        n = Ref{Int64}(0)
+        chk(ctx, ccall((:esph_scene_fluid_count, libesph), Cint, (Ptr{Cvoid}, Ptr{Int64}), scene, n))
+        get(field, w) = begin
+            out = Vector{Float64}(undef, n[] * w)
+            wr = Ref{Int64}(0)
+            chk(ctx, ccall((:esph_scene_get_fluid, libesph), Cint,
+                             (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Ptr{Int64}), scene, field, out, length(out), wr))
+            out
+        end
+        ids = Int.(get(10, 1)) .+ 1
+        keep = fluid[ids]
+        resize!(fluid, length(keep)); fluid .= keep          # deleteat! of the removed ones, order kept
+        X = reshape(get(0, dim), dim, :); V = reshape(get(1, dim), dim, :)
+        R = get(2, 1); P = get(3, 1); C = get(4, 1)
+        T = thermal ? get(7, 1) : Float64[]
+        for (i, q) in enumerate(fluid)
+            q.x_vec_ .= X[:, i]; q.v_vec_ .= V[:, i]
+            q.rho_ = R[i]; q.p_ = P[i]; q.c_ = C[i]
+            thermal && (q.t_ = T[i])
+        end
+    finally
+        ccall((:esph_scene_destroy, libesph), Cint, (Ptr{Cvoid},), scene)
+    end
+    return nothing
+end
+
+# ---- solve! methods (dispatch mirrors Solve/*.jl) ------------------------------------
+function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:WallParticle},
+                            kernel::SmoothKernel, model::WeaklyCompressibleLiquidModel,
+                            td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
+                            check::Union{Nothing,AABBCheck} = nothing)
+    run_gpu!(fluid, [wall], kernel, model, td, Int32(0), check, dev)
+end
+
+function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:StaticVelocityWallParticle},
+                            kernel::SmoothKernel, model::WeaklyCompressibleLiquidModel,
+                            td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
+                            check::Union{Nothing,AABBCheck} = nothing)
+    run_gpu!(fluid, [wall], kernel, model, td, Int32(1), check, dev)
+end
+
+function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:ThermostaticWallParticle},
+                            kernel::SmoothKernel, model::ThermalWeaklyCompressibleLiquidModel,
+                            td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
+                            check::Union{Nothing,AABBCheck} = nothing)
+    run_gpu!(fluid, [wall], kernel, model, td, Int32(2), check, dev)
+end
+
+function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:WallParticle},
+                            thermo::AbstractVector{<:ThermostaticWallParticle}, kernel::SmoothKernel,
+                            model::ThermalWeaklyCompressibleLiquidModel,
+                            td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
+                            check::Union{Nothing,AABBCheck} = nothing)
+    run_gpu!(fluid, [wall, thermo], kernel, model, td, Int32(3), check, dev)
+end
+
+# ---- kernels and neighbour lists on the device -----------------------------------------
+function device_kernel(r::AbstractVector{Float64}, k::SmoothKernel; dev::MI355X = MI355X())
+    ctx = context(dev)
+    w = similar(r); dw = similar(r)
+    chk(ctx, ccall((:esph_kernel_eval, libesph), Cint,
+                     (Ptr{Cvoid}, Int32, Int32, Float64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                     ctx.handle, kernel_kind(k), k.dim_, k.influence_radius_, length(r), r, w, dw))
+    return w, dw
+end
+
+"Pairs (i, j) (1-based) within `cutoff`: within x (i < j), or between x and y."
+function gpu_neighborlist(x::Matrix{Float64}, cutoff::Float64, y::Union{Nothing,Matrix{Float64}} = nothing;
+                          dev::MI355X = MI355X())
+    ctx = context(dev)
+    dim = size(x, 1)
+    ny = y === nothing ? 0 : size(y, 2)
+    yp = y === nothing ? Ptr{Float64}(C_NULL) : pointer(y)
+    m = Ref{Int64}(0)
+    GC.@preserve x y begin
+        chk(ctx, ccall((:esph_neighbor_list, libesph), Cint,
+                         (Ptr{Cvoid}, Int32, Float64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Int32}, Int64, Ptr{Int64}),
+                         ctx.handle, dim, cutoff, size(x, 2), x, ny, yp, C_NULL, 0, m))
+        out = Matrix{Int32}(undef, 2, m[])
+        chk(ctx, ccall((:esph_neighbor_list, libesph), Cint,
+                         (Ptr{Cvoid}, Int32, Float64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Int32}, Int64, Ptr{Int64}),
+                         ctx.handle, dim, cutoff, size(x, 2), x, ny, yp, out, m[], m))
+    end
+    return [(Int(out[1, q]) + 1, Int(out[2, q]) + 1) for q in 1:m[]]
+end
+
+end # module

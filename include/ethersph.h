@@ -225,10 +225,63 @@ int esph_scene_neighbours(esph_scene* scene, int32_t pairset, int32_t* out_pairs
                           int64_t capacity, int64_t* m);
 /* Average device time per step of each phase over the steps since the last
    call (milliseconds): [0] cell sort, [1] continuity+EOS, [2] forces+integrate,
-   [3] Shepard (per step it ran), [4] check; and the number of steps timed.
+   [3] Shepard (per step it ran), [4] 0 (the check runs inside the sort);
+   and the number of steps timed.
    Timing is off unless enabled (it adds events to the stream). */
 int esph_scene_enable_timing(esph_scene* scene, int32_t enable);
 int esph_scene_phase_times(esph_scene* scene, double* ms5, int64_t* steps);
+/* out4: [0] particle-steps whose neighbour count exceeded the per-particle
+   list width (those particles scanned their cells instead; results are the
+   same), [1] fluid-fluid list width, [2] fluid-wall list width, [3] cells. */
+int esph_scene_stats(esph_scene* scene, int64_t* out4);
+/* Graph mode: the first step captures one step as a HIP graph; later steps
+   replay it (the Shepard pass is then launched every step and skips itself
+   on the device when not due).  Phase timing is not recorded in graph mode. */
+int esph_scene_enable_graph(esph_scene* scene, int32_t enable);
+
+/* ---- batches (ensembles) ---------------------------------------------------
+   A batch advances several scenes of one context together: one launch per
+   (dim, kernel, precision) class and phase for all members, driven by a
+   device table of scenes and tile tables (scene, first particle).  Members
+   keep their state and may still be stepped alone; the result of a scene is
+   the same alone or in a batch.  Members must stay alive while the batch
+   exists. */
+typedef struct esph_batch esph_batch;
+int esph_batch_create(esph_ctx* ctx, int32_t nscenes, esph_scene* const* scenes, esph_batch** out);
+int esph_batch_destroy(esph_batch* batch);
+int esph_batch_enable_graph(esph_batch* batch, int32_t enable);
+int esph_batch_step(esph_batch* batch, int64_t nsteps);
+int esph_batch_sync(esph_batch* batch);
+
+/* ---- serving: continuous batching of simulation requests -------------------
+   No counterpart in the reference (which runs one solve! at a time).
+   esph_server_submit copies a request (descriptor, walls, fluid, number of
+   steps) into a FIFO queue.  esph_server_advance admits queued requests in
+   arrival order while the context's HBM budget (esph_ctx_create) and
+   max_active allow, advances every running request together as one batch
+   (graph-replayed) by min(max_steps, fewest steps left among them), then reads
+   back and frees finished requests (their x, v, rho, p, c, id [, t] stay
+   available through esph_server_result until released).  A request that
+   can never fit the budget fails.  Not thread-safe: one host thread drives a
+   server. */
+enum { ESPH_REQ_QUEUED = 0, ESPH_REQ_RUNNING = 1, ESPH_REQ_DONE = 2, ESPH_REQ_FAILED = 3 };
+typedef struct esph_server esph_server;
+int esph_server_create(esph_ctx* ctx, int32_t max_active, esph_server** out); /* max_active <= 0: no limit */
+int esph_server_destroy(esph_server* server);
+int esph_server_submit(esph_server* server, const esph_scene_desc* desc, int32_t nwall_groups,
+                       const esph_wall_arrays* walls, const esph_fluid_arrays* fluid, int64_t nsteps,
+                       int64_t* request_id);
+int esph_server_advance(esph_server* server, int64_t max_steps, int64_t* stepped);
+int esph_server_status(esph_server* server, int64_t request_id, int32_t* state, int64_t* steps_done);
+/* Fields of a finished request in reference order (ESPH_FIELD_X, _V, _RHO, _P,
+   _C, _ID, and _T for thermal solvers); out may be NULL to query *written. */
+int esph_server_result(esph_server* server, int64_t request_id, int32_t field, double* out, int64_t capacity,
+                       int64_t* written);
+int esph_server_release(esph_server* server, int64_t request_id);
+/* out8: [0] p50 and [1] p99 step latency (ms, over every scene-step),
+   [2] completed requests, [3] completed per second since the first submit,
+   [4] running, [5] queued, [6] device bytes held, [7] batch steps run. */
+int esph_server_stats(esph_server* server, double* out8);
 
 #ifdef __cplusplus
 }

@@ -301,3 +301,18 @@ def test_solve_dropin_particle_lists(ctx):
     assert len(fluid) == ref.fluid_count() < 144
     X = np.array([p.x_vec_ for p in fluid])
     assert rel_l2(X, ref.get(_lib.FIELD_X)) < 1e-9
+
+
+def test_overflow_scan_path_matches_oracle(ctx):
+    """Particles packed at 0.45 of the lattice spacing overflow the neighbour
+    lists (sized from the gap); they take the cell-scan path."""
+    sc = random_blob(n=900, kernel="WendlandC2", reinit=5, spacing=0.01)
+    sc.fluid.x_vec_ = sc.fluid.x_vec_ * 0.45
+    say("overflow path")
+    dev = sc.make_scene(_lib.FP64, ctx)
+    ref = O.from_scenario(sc)
+    dev.step(20)
+    dev.sync()
+    ref.step(20)
+    assert dev.stats()["list_overflow"] > 0
+    compare_fields(dev, ref, 1e-9)
