@@ -175,6 +175,7 @@ static void free_scene_device(esph_scene* s) {
 
 extern "C" int esph_scene_destroy(esph_scene* s) {
     if (!s) return ESPH_ERR_ARG;
+    if (s->batch_refs > 0) return fail(s->ctx, ESPH_ERR_STATE, "scene_destroy: the scene is a member of a live batch");
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     free_scene_device(s);
@@ -223,6 +224,33 @@ extern "C" int esph_scene_set_walls(esph_scene* s, int32_t ngroups, const esph_w
     return ESPH_OK;
 }
 
+extern "C" int esph_scene_set_buffers(esph_scene* s, const esph_fluid_arrays* inlet, const esph_fluid_arrays* outlet) {
+    if (!s || !inlet || !outlet) return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "set_buffers: null argument");
+    if (s->d.solver != ESPH_SOLVER_FLUID_INLET_OUTLET_WALL)
+        return fail(s->ctx, ESPH_ERR_ARG, "set_buffers: only the inlet/outlet solver has buffers");
+    if (s->allocated) return fail(s->ctx, ESPH_ERR_STATE, "set_buffers must precede the first set_fluid");
+    const int D = s->d.dim;
+    for (const esph_fluid_arrays* b : {inlet, outlet})
+        if (b->n < 0 || (b->n > 0 && (!b->x || !b->v || !b->rho || !b->p || !b->c || !b->mass || !b->gap)))
+            return fail(s->ctx, ESPH_ERR_ARG, "set_buffers: missing arrays");
+    if (inlet->n + outlet->n > INT32_MAX / 4) return fail(s->ctx, ESPH_ERR_ARG, "set_buffers: too many particles");
+    s->n_inlet = inlet->n;
+    s->n_outlet = outlet->n;
+    auto cat = [&](std::vector<double>& dst, const double* a, const double* b, int w) {
+        dst.resize((size_t)(inlet->n + outlet->n) * w);
+        if (inlet->n) std::memcpy(dst.data(), a, sizeof(double) * inlet->n * w);
+        if (outlet->n) std::memcpy(dst.data() + inlet->n * w, b, sizeof(double) * outlet->n * w);
+    };
+    cat(s->bx, inlet->x, outlet->x, D);
+    cat(s->bv, inlet->v, outlet->v, D);
+    cat(s->brho, inlet->rho, outlet->rho, 1);
+    cat(s->bp, inlet->p, outlet->p, 1);
+    cat(s->bc, inlet->c, outlet->c, 1);
+    cat(s->bm, inlet->mass, outlet->mass, 1);
+    cat(s->bgap, inlet->gap, outlet->gap, 1);
+    return ESPH_OK;
+}
+
 // Cell grid: cells 1.0001 R wide over the bounding box of fluid, walls and
 // finite check bounds (or the caller's domain), one cell of padding.
 static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) {
@@ -240,6 +268,7 @@ static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) 
     } else {
         add(f->x, f->n);
         if (s->walls.n) add(s->walls.x.data(), s->walls.n);
+        if (!s->bx.empty()) add(s->bx.data(), s->n_inlet + s->n_outlet);
         if (s->d.check_enabled)
             for (int k = 0; k < D; ++k) {
                 if (std::isfinite(s->d.check_min[k])) lo[k] = std::min(lo[k], s->d.check_min[k]);
@@ -282,8 +311,11 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     esph_ctx* ctx = s->ctx;
     const esph_scene_desc& d = s->d;
     const int D = d.dim;
-    int64_t cap64 = d.fluid_capacity > 0 ? d.fluid_capacity : f->n;
-    if (cap64 < f->n) return fail(ctx, ESPH_ERR_ARG, "fluid_capacity smaller than the upload");
+    const bool io = d.solver == ESPH_SOLVER_FLUID_INLET_OUTLET_WALL;
+    const int64_t n_all = f->n + s->n_inlet + s->n_outlet;  // buffers share the particle arrays
+    // the inlet/outlet solver grows by inlet copies: default headroom of 2x
+    int64_t cap64 = d.fluid_capacity > 0 ? d.fluid_capacity : (io ? 2 * n_all + 1024 : n_all);
+    if (cap64 < n_all) return fail(ctx, ESPH_ERR_ARG, "fluid_capacity smaller than the upload");
     if (cap64 > INT32_MAX / 4) return fail(ctx, ESPH_ERR_ARG, "fluid capacity too large");
     const int cap = (int)std::max<int64_t>(cap64, 1);
     SceneDev hd{};
@@ -293,8 +325,23 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.prec = d.precision;
     const bool thermal = d.solver == ESPH_SOLVER_FLUID_THERMOSTATIC_WALL ||
                          d.solver == ESPH_SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL;
-    hd.flags = (thermal ? FLAG_THERMAL : 0) | (d.solver != ESPH_SOLVER_FLUID_WALL ? FLAG_WALL_PRESSURE : 0) |
-               (d.check_enabled ? FLAG_CHECK : 0);
+    // wall pressure: every solver but FluidWallSolve and the inlet/outlet one
+    // (FluidWallSolve.jl:68-79, FluidInletOutletWallSolve.jl:121-122)
+    const bool wall_p = d.solver != ESPH_SOLVER_FLUID_WALL && !io;
+    hd.flags = (thermal ? FLAG_THERMAL : 0) | (wall_p ? FLAG_WALL_PRESSURE : 0) | (d.check_enabled && !io ? FLAG_CHECK : 0) |
+               (io ? FLAG_IO : 0);
+    hd.io_axis = d.io_axis;
+    hd.inlet_enter = d.inlet_enter;
+    hd.inlet_shift = d.inlet_shift;
+    hd.outlet_enter = d.outlet_enter;
+    hd.outlet_exit = d.outlet_exit;
+    hd.profile = d.profile;
+    hd.profile_terms = d.profile_terms;
+    hd.outlet_profile = d.outlet_profile;
+    hd.profile_f = d.profile_f;
+    hd.profile_nu = d.profile_nu;
+    hd.profile_l = d.profile_l;
+    for (int k = 0; k < 3; ++k) hd.profile_c[k] = d.profile_center[k];
     hd.gamma = d.gamma;
     hd.cap = cap;
     hd.kreinit = d.density_reinitialized_interval;
@@ -368,6 +415,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         S.m = (double*)dal(G * cap);
         S.gap = (double*)dal(G * cap);
         S.id = (int32_t*)dal(I * cap);
+        S.kind = io ? (int8_t*)dal(sizeof(int8_t) * cap) : nullptr;
         if (thermal) {
             S.T = (double*)dal(G * cap);
             S.kappa = (double*)dal(G * cap);
@@ -393,6 +441,14 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.wnbr = (int32_t*)dal(I * (size_t)cap * std::max(hd.kfw, 1));
     hd.nnbw = (int32_t*)dal(I * (size_t)cap);
     hd.ovf = (int32_t*)dal(I * (size_t)cap);
+    if (io) {
+        hd.spawn_src = (int32_t*)dal(I * (size_t)cap);
+        hd.spawn_id = (int32_t*)dal(I * (size_t)cap);
+        hd.spawn_key = (int32_t*)dal(I * (size_t)cap);
+        hd.spawn_slot = (int32_t*)dal(I * (size_t)cap);
+        hd.spawn_order = (int32_t*)dal(I * (size_t)cap);
+        hd.leave_id = (int32_t*)dal(I * (size_t)cap);
+    }
     // walls, binned on the host once (stable by upload index)
     hd.wx = (double*)dal(G * std::max(nw, 1) * D);
     hd.wn = (double*)dal(G * std::max(nw, 1) * D);
@@ -525,26 +581,41 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
         HIPCHK(ctx, hipMemcpy(s->dev, &s->h, sizeof(SceneDev), hipMemcpyHostToDevice));
     }
     SceneDev& hd = s->h;
-    if (f->n > hd.cap) return fail(ctx, ESPH_ERR_CAPACITY, "set_fluid: more particles than the scene capacity");
-    const int n = (int)f->n;
+    const bool io = (hd.flags & FLAG_IO) != 0;
+    const int64_t nb = io ? s->n_inlet + s->n_outlet : 0;
+    if (f->n + nb > hd.cap) return fail(ctx, ESPH_ERR_CAPACITY, "set_fluid: more particles than the scene capacity");
+    const int nf = (int)f->n;
+    const int n = (int)(f->n + nb);
     hipStream_t st = ctx->stream;
     HIPCHK(ctx, hipStreamSynchronize(st));
     const size_t G = sizeof(double);
     FluidSet& S = hd.set[0];
+    // fluid, then inlet and outlet buffers; ids are sequence numbers per group
     std::vector<int32_t> ids(n);
-    std::iota(ids.begin(), ids.end(), 0);
+    std::vector<int8_t> kinds(n, (int8_t)KIND_FLUID);
+    for (int i = 0; i < nf; ++i) ids[i] = i;
+    for (int64_t i = 0; i < s->n_inlet; ++i) { ids[nf + i] = (int32_t)i; kinds[nf + i] = KIND_INLET; }
+    for (int64_t i = 0; i < s->n_outlet; ++i) {
+        ids[nf + s->n_inlet + i] = (int32_t)i;
+        kinds[nf + s->n_inlet + i] = KIND_OUTLET;
+    }
     hipError_t e = hipSuccess;
     auto up = [&](void* dst, const void* src, size_t bytes) {
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
     };
-    up(hd.xn, f->x, G * n * D);
-    up(hd.vn, f->v, G * n * D);
-    up(hd.rhon, f->rho, G * n);
-    up(S.p, f->p, G * n);
-    up(S.c, f->c, G * n);
-    up(S.m, f->mass, G * n);
-    up(S.gap, f->gap, G * n);
+    auto up2 = [&](double* dst, const double* a, int64_t na, const std::vector<double>& b, int w) {
+        up(dst, a, G * na * w);
+        if (nb) up(dst + na * w, b.data(), G * nb * w);
+    };
+    up2(hd.xn, f->x, nf, s->bx, D);
+    up2(hd.vn, f->v, nf, s->bv, D);
+    up2(hd.rhon, f->rho, nf, s->brho, 1);
+    up2(S.p, f->p, nf, s->bp, 1);
+    up2(S.c, f->c, nf, s->bc, 1);
+    up2(S.m, f->mass, nf, s->bm, 1);
+    up2(S.gap, f->gap, nf, s->bgap, 1);
     up(S.id, ids.data(), sizeof(int32_t) * n);
+    if (io) up(S.kind, kinds.data(), sizeof(int8_t) * n);
     if (thermal) {
         up(hd.Tn, f->t, G * n);
         up(S.kappa, f->kappa, G * n);
@@ -554,6 +625,8 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
     dyn.step = 0;
     dyn.parity = 0;
     dyn.n_live = n;
+    dyn.next_fluid_id = nf;
+    dyn.next_outlet_id = (int32_t)s->n_outlet;
     up(hd.dyn, &dyn, sizeof(SceneDyn));
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(ctx, "set_fluid upload", e);
@@ -635,6 +708,8 @@ extern "C" int esph_scene_sync(esph_scene* s) {
     SceneDyn dyn;
     HIPCHK(ctx, hipMemcpy(&dyn, s->h.dyn, sizeof(SceneDyn), hipMemcpyDeviceToHost));
     if (dyn.err & 1) return fail(ctx, ESPH_ERR_NAN, "a non-finite fluid position was met");
+    if (dyn.err & 2)
+        return fail(ctx, ESPH_ERR_CAPACITY, "inlet copies exceeded the scene capacity (fluid_capacity) and were dropped");
     return ESPH_OK;
 }
 
@@ -655,6 +730,7 @@ static int read_dyn(esph_scene* s, SceneDyn* dyn) {
 extern "C" int esph_scene_fluid_count(esph_scene* s, int64_t* n) {
     if (!s || !n) return ESPH_ERR_ARG;
     if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    if (s->h.flags & FLAG_IO) return esph_scene_group_count(s, ESPH_GROUP_FLUID, n);
     SceneDyn dyn;
     int rc = read_dyn(s, &dyn);
     if (rc) return rc;
@@ -662,16 +738,45 @@ extern "C" int esph_scene_fluid_count(esph_scene* s, int64_t* n) {
     return ESPH_OK;
 }
 
-extern "C" int esph_scene_get_fluid(esph_scene* s, int32_t field, double* out, int64_t capacity, int64_t* written) {
+// Live particles of one group (fluid, inlet or outlet buffer) in reference
+// order: ascending id within the group.  Scenes without buffers hold fluid
+// only.
+static int group_order(esph_scene* s, int32_t group, SceneDyn& dyn, std::vector<int32_t>& order,
+                       std::vector<int32_t>& ids) {
+    int rc = read_dyn(s, &dyn);
+    if (rc) return rc;
+    esph_ctx* ctx = s->ctx;
+    const int n = dyn.n_live;
+    const FluidSet& S = s->h.set[dyn.parity];
+    std::vector<int32_t> all(n);
+    std::vector<int8_t> kinds;
+    if (n) HIPCHK(ctx, hipMemcpy(all.data(), S.id, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (S.kind && n) {
+        kinds.resize(n);
+        HIPCHK(ctx, hipMemcpy(kinds.data(), S.kind, sizeof(int8_t) * n, hipMemcpyDeviceToHost));
+    }
+    order.clear();
+    for (int q = 0; q < n; ++q)
+        if ((kinds.empty() ? (int)KIND_FLUID : (int)kinds[q]) == group) order.push_back(q);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return all[a] < all[b]; });
+    ids.resize(order.size());
+    for (size_t q = 0; q < order.size(); ++q) ids[q] = all[order[q]];
+    return ESPH_OK;
+}
+
+static int get_group(esph_scene* s, int32_t group, int32_t field, double* out, int64_t capacity, int64_t* written) {
     if (!s || !out) return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "get_fluid: null argument");
     if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    if (group != ESPH_GROUP_FLUID && group != ESPH_GROUP_INLET && group != ESPH_GROUP_OUTLET)
+        return fail(s->ctx, ESPH_ERR_ARG, "get_group: bad group");
     esph_ctx* ctx = s->ctx;
     const int D = s->d.dim;
     const bool thermal = s->h.flags & FLAG_THERMAL;
     SceneDyn dyn;
-    int rc = read_dyn(s, &dyn);
+    std::vector<int32_t> order, ids;
+    int rc = group_order(s, group, dyn, order, ids);
     if (rc) return rc;
-    const int n = dyn.n_live;
+    const int n = (int)order.size();
     const FluidSet& S = s->h.set[dyn.parity];
     const double* src = nullptr;
     int w = 1;
@@ -692,20 +797,37 @@ extern "C" int esph_scene_get_fluid(esph_scene* s, int32_t field, double* out, i
     if (field != ESPH_FIELD_ID && !src) return fail(ctx, ESPH_ERR_ARG, "get_fluid: field not present for this solver");
     if (written) *written = (int64_t)n * w;
     if (capacity < (int64_t)n * w) return fail(ctx, ESPH_ERR_CAPACITY, "get_fluid: output too small");
-    std::vector<int32_t> ids(n);
-    HIPCHK(ctx, hipMemcpy(ids.data(), S.id, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-    // reference order = ascending upload index among live particles
-    std::vector<int32_t> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return ids[a] < ids[b]; });
     if (field == ESPH_FIELD_ID) {
-        for (int q = 0; q < n; ++q) out[q] = (double)ids[order[q]];
+        for (int q = 0; q < n; ++q) out[q] = (double)ids[q];
         return ESPH_OK;
     }
-    std::vector<double> buf((size_t)n * w);
-    HIPCHK(ctx, hipMemcpy(buf.data(), src, sizeof(double) * n * w, hipMemcpyDeviceToHost));
+    const int nl = dyn.n_live;
+    std::vector<double> buf((size_t)nl * w);
+    if (nl) HIPCHK(ctx, hipMemcpy(buf.data(), src, sizeof(double) * nl * w, hipMemcpyDeviceToHost));
     for (int q = 0; q < n; ++q)
         for (int k = 0; k < w; ++k) out[(size_t)q * w + k] = buf[(size_t)order[q] * w + k];
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_get_fluid(esph_scene* s, int32_t field, double* out, int64_t capacity, int64_t* written) {
+    return get_group(s, ESPH_GROUP_FLUID, field, out, capacity, written);
+}
+
+extern "C" int esph_scene_get_group(esph_scene* s, int32_t group, int32_t field, double* out, int64_t capacity,
+                                    int64_t* written) {
+    return get_group(s, group, field, out, capacity, written);
+}
+
+extern "C" int esph_scene_group_count(esph_scene* s, int32_t group, int64_t* n) {
+    if (!s || !n) return ESPH_ERR_ARG;
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    if (group != ESPH_GROUP_FLUID && group != ESPH_GROUP_INLET && group != ESPH_GROUP_OUTLET)
+        return fail(s->ctx, ESPH_ERR_ARG, "group_count: bad group");
+    SceneDyn dyn;
+    std::vector<int32_t> order, ids;
+    int rc = group_order(s, group, dyn, order, ids);
+    if (rc) return rc;
+    *n = (int64_t)order.size();
     return ESPH_OK;
 }
 
@@ -736,20 +858,20 @@ static int export_pairs(esph_scene* s, int pairset, std::vector<int64_t>& pairs_
     if (n) e = hipMemcpyAsync(offs, ho.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_pairs(b, pairset, nullptr, offs, dout, st);
     if (e == hipSuccess && m) e = hipMemcpyAsync(hp.data(), dout, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost, st);
-    std::vector<int32_t> ids(n);
-    if (e == hipSuccess && n) e = hipMemcpyAsync(ids.data(), s->h.set[dyn.parity].id, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     a.release();
     if (e != hipSuccess) return hip_fail(ctx, "export write", e);
-    // upload index -> reference index (rank among live upload indices)
-    std::vector<int32_t> sorted_ids(ids);
-    std::sort(sorted_ids.begin(), sorted_ids.end());
-    std::vector<int32_t> rank(s->n_upload > 0 ? s->n_upload : 1, -1);
-    for (int q = 0; q < n; ++q) rank[sorted_ids[q]] = q;
+    // fluid id -> reference index (rank among the live fluid ids)
+    std::vector<int32_t> order, sorted_ids;
+    rc = group_order(s, ESPH_GROUP_FLUID, dyn, order, sorted_ids);
+    if (rc) return rc;
+    auto rank = [&](int32_t id) -> int64_t {
+        return std::lower_bound(sorted_ids.begin(), sorted_ids.end(), id) - sorted_ids.begin();
+    };
     pairs_out.resize(m);
     for (int64_t q = 0; q < m; ++q) {
-        const int64_t i = rank[hp[2 * q]];
-        const int64_t j = pairset == ESPH_PAIRS_FLUID_FLUID ? rank[hp[2 * q + 1]] : hp[2 * q + 1];
+        const int64_t i = rank(hp[2 * q]);
+        const int64_t j = pairset == ESPH_PAIRS_FLUID_FLUID ? rank(hp[2 * q + 1]) : hp[2 * q + 1];
         pairs_out[q] = (i << 32) | j;
     }
     std::sort(pairs_out.begin(), pairs_out.end());

@@ -153,6 +153,7 @@ extern "C" int esph_batch_create(esph_ctx* ctx, int32_t nscenes, esph_scene* con
         delete b;
         return rc;
     }
+    for (esph_scene* s : m) s->batch_refs++;
     *out = b;
     return ESPH_OK;
 }
@@ -162,6 +163,7 @@ extern "C" int esph_batch_destroy(esph_batch* b) {
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     b->plan.release();
+    for (esph_scene* s : b->members) s->batch_refs--;
     delete b;
     return ESPH_OK;
 }

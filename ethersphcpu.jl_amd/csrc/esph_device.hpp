@@ -20,7 +20,11 @@ enum : int {
     FLAG_THERMAL = 1,        // thermal fluid + ThermalWeaklyCompressibleLiquidModel
     FLAG_WALL_PRESSURE = 2,  // pressureForce!(fluid, wall) in the wall loop
     FLAG_CHECK = 4,          // AABB deletion predicate enabled
+    FLAG_IO = 8,             // inlet/outlet solver: buffer particles and migration
 };
+
+// Particle kinds of the inlet/outlet solver (FluidSet::kind).
+enum : int8_t { KIND_FLUID = 0, KIND_INLET = 1, KIND_OUTLET = 2 };
 
 // Per-scene state that changes every step and lives in device memory, so a
 // captured step graph can be replayed without the host.
@@ -31,6 +35,10 @@ struct SceneDyn {
     int32_t err;     // bit 0: a non-finite position was met
     int32_t overflow;  // particle-steps whose neighbour count exceeded the list width
     int32_t novf;      // this step's overflowed particles (listed in SceneDev::ovf)
+    int32_t nspawn;    // inlet/outlet: inlet particles entering the fluid this step
+    int32_t nleave;    // inlet/outlet: fluid particles entering the outlet this step
+    int32_t next_fluid_id;   // next sequence number of the fluid group
+    int32_t next_outlet_id;  // next sequence number of the outlet group
     int32_t pad;
 };
 
@@ -52,7 +60,8 @@ struct FluidSet {
     double* T;
     double* kappa;
     double* cp;
-    int32_t* id;    // upload index
+    int32_t* id;    // sequence number within the particle's group (upload index at first)
+    int8_t* kind;   // KIND_* (inlet/outlet solver only; null otherwise)
     void* A;
     void* B;
     void* kap;
@@ -117,6 +126,16 @@ struct SceneDev {
     void* wA;
     void* wB;
     void* wkap;
+    // inlet/outlet solver: migration planes along io_axis and buffer profile
+    int32_t io_axis, profile, profile_terms, outlet_profile;
+    double inlet_enter, inlet_shift, outlet_enter, outlet_exit;
+    double profile_f, profile_nu, profile_l, profile_c[3];
+    int32_t* spawn_src;   // per spawn (append order): source inlet particle
+    int32_t* spawn_id;    //   its inlet sequence number
+    int32_t* spawn_key;   //   cell of the copy
+    int32_t* spawn_slot;  //   slot in that cell's count
+    int32_t* spawn_order; // spawn index by rank (rank = order of the inlet sequence numbers)
+    int32_t* leave_id;    // fluid sequence numbers of this step's fluid -> outlet moves
     SceneDyn* dyn;
 };
 

@@ -219,6 +219,13 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
         const SceneDyn dyn = *s.dyn;
         for_work<SCAN>(s, dyn, tile, [&](int i) {
         const FluidSet& S = s.set[dyn.parity];
+        if (S.kind && S.kind[i] != KIND_FLUID) {
+            // inlet/outlet buffers keep rho and p (drho is reset,
+            // FluidInletOutletWallSolve.jl:91-99) and need no lists
+            s.nnb[i] = 0;
+            if (s.nw > 0) s.nnbw[i] = 0;
+            return;
+        }
         const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
         double xi[3];
         load_x<D>(S.x, i, xi);
@@ -294,9 +301,27 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
 // and 1 / W(gap) is taken per particle when both gaps are equal.
 // New x, v, T go to xn, vn, Tn in the current particle order.
 // ---------------------------------------------------------------------------
+// Buffer particles of the inlet/outlet solver: dv = 0, then updatePosition!
+// (FluidInletOutletWallSolve.jl:170-186): x += v dt - 0 dt^2 / 2.
+template <int D>
+__device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S, int i) {
+#pragma clang fp contract(off)
+    const double dt = s.dt;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double v = S.v[i * D + k];
+        s.vn[i * D + k] = v;
+        s.xn[i * D + k] = S.x[i * D + k] + (v * dt - 0.0 * (dt * dt) / 2.0);
+    }
+}
+
 template <int D, int K, typename T, bool THERMAL, bool SCAN>
 __device__ __forceinline__ void forces_body(const SceneDev& s, const FluidSet& S, int i) {
     using KQ = KernelQ<T, K>;
+    if (S.kind && S.kind[i] != KIND_FLUID) {
+        buffer_move<D>(s, S, i);
+        return;
+    }
     const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
     const T eta2 = (T)(0.01 * (s.h * s.h));
     const T nu0 = (T)s.nu0;
@@ -462,10 +487,13 @@ __global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__
         for_work<SCAN>(s, dyn, tile, [&](int i) {
         if (!SCAN && overflowed(s, i)) return;
         const FluidSet& S = s.set[dyn.parity];
+        const int8_t* kind = S.kind;
+        if (kind && kind[i] != KIND_FLUID) return;
         double xi[3];
         load_x<D>(S.x, i, xi);
         double sw = 0.0, ss = 0.0;
         for_pairs<D, SCAN>(s, s.cell_start, S.x, xi, i, i, s.nbr, s.nnb[i], [&](int j, double d2, const double*) {
+            if (kind && kind[j] != KIND_FLUID) return;  // fluid-fluid pairs only
             double W, dW;
             KQ::wdw(sqrt(d2) * s.inv_h, s.kA, s.kB, W, dW);
             const double rho_j = S.rho[j];
@@ -510,6 +538,19 @@ __global__ __launch_bounds__(kBlock) void k_sort_clear(const SceneDev* __restric
             if (c <= s.ncell) s.count[c] = 0;
         }
     }
+}
+
+__device__ __forceinline__ int cell_key(const SceneDev& s, int D, const double* x) {
+    const int cx = cell_coord(x[0], s.lo[0], s.inv_cs, s.nc[0]);
+    const int cy = cell_coord(x[1], s.lo[1], s.inv_cs, s.nc[1]);
+    const int cz = D == 3 ? cell_coord(x[2], s.lo[2], s.inv_cs, s.nc[2]) : 0;
+    return cell_linear(s, cx, cy, cz);
+}
+
+// Spawned copies valid this step (the rest exceeded the capacity).
+__device__ __forceinline__ int spawn_count(const SceneDev& s, const SceneDyn& d) {
+    const int room = s.cap - d.n_live;
+    return d.nspawn < room ? d.nspawn : room;
 }
 
 // Slot of this lane's key in count[key]: lanes of one wavefront holding the
@@ -558,18 +599,33 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
             for (int k = 0; k < D; ++k)
                 if (x[k] > s.check_max[k] || x[k] < s.check_min[k]) dead = true;
         }
-        if (dead) {
-            key = s.ncell;
-        } else {
-            int c[3];
-            c[0] = cell_coord(x[0], s.lo[0], s.inv_cs, s.nc[0]);
-            c[1] = cell_coord(x[1], s.lo[1], s.inv_cs, s.nc[1]);
-            c[2] = D == 3 ? cell_coord(x[2], s.lo[2], s.inv_cs, s.nc[2]) : 0;
-            key = cell_linear(s, c[0], c[1], c[2]);
+        // inlet/outlet migration (FluidInletOutletWallSolve.jl:196-262)
+        int spawn_key = -1;
+        if (mode == 0 && (s.flags & FLAG_IO)) {
+            const FluidSet& S = s.set[dyn.parity];
+            const int kd = S.kind[i], a = s.io_axis;
+            if (kd == KIND_OUTLET && x[a] > s.outlet_exit) dead = true;
+            if (kd == KIND_FLUID && x[a] > s.outlet_enter) s.leave_id[atomicAdd(&s.dyn->nleave, 1)] = S.id[i];
+            if (kd == KIND_INLET && x[a] > s.inlet_enter) {
+                spawn_key = cell_key(s, D, x);  // the fluid copy stays where the inlet particle is
+                x[a] -= s.inlet_shift;          // the inlet particle moves back
+            }
         }
+        key = dead ? s.ncell : cell_key(s, D, x);
         if (!finite) atomicOr(&s.dyn->err, 1);
         s.key[i] = key;
         s.slot[i] = wave_counted_slot(s.count, key);
+        if (spawn_key >= 0) {
+            const int e = atomicAdd(&s.dyn->nspawn, 1);
+            if (dyn.n_live + e < s.cap) {
+                s.spawn_src[e] = i;
+                s.spawn_id[e] = s.set[dyn.parity].id[i];
+                s.spawn_key[e] = spawn_key;
+                s.spawn_slot[e] = atomicAdd(&s.count[spawn_key], 1);
+            } else {
+                atomicOr(&s.dyn->err, 2);  // capacity exhausted: the copy is dropped
+            }
+        }
     }
 }
 
@@ -664,62 +720,151 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan3(const SceneDev* __restric
 }
 
 __global__ __launch_bounds__(kBlock) void k_sort_place(const SceneDev* __restrict__ scenes,
-                                                       const Tile* __restrict__ tiles, int ntiles) {
+                                                       const Tile* __restrict__ tiles, int ntiles, int mode) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const int i = tile.begin + (int)threadIdx.x;
-        if (i >= s.dyn->n_live) continue;
-        s.tmp[s.cell_start[s.key[i]] + s.slot[i]] = i;
+        const SceneDyn dyn = *s.dyn;
+        if (i < dyn.n_live) {
+            s.tmp[s.cell_start[s.key[i]] + s.slot[i]] = i;
+        } else if (mode == 0 && (s.flags & FLAG_IO) && i < dyn.n_live + spawn_count(s, dyn)) {
+            // a fluid copy of an entering inlet particle; its rank among this
+            // step's copies follows the inlet order (push! in that order)
+            const int e = i - dyn.n_live, ns = spawn_count(s, dyn), my = s.spawn_id[e];
+            int r = 0;
+            for (int q = 0; q < ns; ++q) r += s.spawn_id[q] < my ? 1 : 0;
+            s.spawn_order[r] = e;
+            s.tmp[s.cell_start[s.spawn_key[e]] + s.spawn_slot[e]] = dyn.n_live + r;
+        }
     }
 }
 
+// Velocity profile of the inlet/outlet buffers (the examples'
+// user_defined_function): PoiseuilleFlow2D.jl:28-35, PoiseuilleFlow3D.jl:32-34.
+__device__ double profile_u(const SceneDev& s, const double* x, double t) {
+#pragma clang fp contract(off)
+    const double f = s.profile_f, nu = s.profile_nu, l = s.profile_l;
+    if (s.profile == ESPH_PROFILE_CHANNEL_2D) {
+        const double PI = 3.14159265358979323846;
+        const double y = x[1] - s.profile_c[1];
+        double ux = f / 2.0 / nu * y * (l - y);
+        for (int n = 0; n <= s.profile_terms; ++n) {
+            const double o = (double)(2 * n + 1);
+            ux -= 4.0 * f * (l * l) / nu / (PI * PI * PI) / (o * o * o) * sin(PI * y / l * o) *
+                  exp(-(o * o) * (PI * PI) * nu * t / (l * l));
+        }
+        return ux;
+    }
+    if (s.profile == ESPH_PROFILE_PIPE_3D) {
+        const double y = x[1] - s.profile_c[1], z = x[2] - s.profile_c[2];
+        return f / 4.0 / nu * (l * l - y * y - z * z);
+    }
+    return 0.0;
+}
+
 // Final position = cell start + number of cell members with a smaller
-// previous index; gathers every field of particle i into the other set.
+// entry index (previous order; a step's fluid copies of inlet particles come
+// after every existing particle, in inlet order); gathers every field into
+// the other set.  With the inlet/outlet solver it also applies the step's
+// migration (kinds and group sequence numbers) and the buffer velocities.
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restrict__ scenes,
                                                          const Tile* __restrict__ tiles, int ntiles, int mode) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
-        const int i = tile.begin + (int)threadIdx.x;
-        if (i >= dyn.n_live) continue;
-        const int key = s.key[i];
-        if (key == s.ncell) continue;  // deleted
+        const bool io = mode == 0 && (s.flags & FLAG_IO);
+        const int ns = io ? spawn_count(s, dyn) : 0;
+        const int q = tile.begin + (int)threadIdx.x;
+        if (q >= dyn.n_live + ns) continue;
+        const bool spawn = q >= dyn.n_live;
+        int src, key, sp_rank = 0;
+        if (spawn) {
+            sp_rank = q - dyn.n_live;
+            const int e = s.spawn_order[sp_rank];
+            src = s.spawn_src[e];
+            key = s.spawn_key[e];
+        } else {
+            src = q;
+            key = s.key[q];
+            if (key == s.ncell) continue;  // deleted
+        }
         const int b = s.cell_start[key], e = s.cell_start[key + 1];
         int rank = 0;
-        for (int q = b; q < e; ++q) rank += s.tmp[q] < i ? 1 : 0;
+        for (int w = b; w < e; ++w) rank += s.tmp[w] < q ? 1 : 0;
         const int p = b + rank;
         const FluidSet& S = s.set[dyn.parity];
         const FluidSet& O = s.set[dyn.parity ^ 1];
         const int D = s.dim;
         const bool thermal = (s.flags & FLAG_THERMAL) != 0;
-        double v[3] = {0.0, 0.0, 0.0};
+        double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < D; ++k) {
-            v[k] = s.vn[i * D + k];
-            O.x[p * D + k] = s.xn[i * D + k];
+            x[k] = s.xn[src * D + k];
+            v[k] = s.vn[src * D + k];
+        }
+        int8_t kd = S.kind ? S.kind[src] : (int8_t)KIND_FLUID;
+        const bool src_fluid = kd == KIND_FLUID;
+        int32_t id = S.id[src];
+        if (io) {
+            const int a = s.io_axis;
+            if (spawn) {
+                kd = KIND_FLUID;  // push!(fluid, deepcopy(inlet particle))
+                id = dyn.next_fluid_id + sp_rank;
+            } else if (kd == KIND_INLET && x[a] > s.inlet_enter) {
+                x[a] -= s.inlet_shift;  // inlet_into_calculation_domain_modify
+            } else if (kd == KIND_FLUID && x[a] > s.outlet_enter) {
+                int r = 0;  // push! onto the outlet in fluid order
+                for (int l = 0; l < dyn.nleave; ++l) r += s.leave_id[l] < id ? 1 : 0;
+                kd = KIND_OUTLET;
+                id = dyn.next_outlet_id + r;
+            }
+            if (kd != KIND_FLUID && s.profile != ESPH_PROFILE_NONE) {
+                const double tt = (double)(dyn.step + 1) * s.dt;
+                if (kd == KIND_INLET || s.outlet_profile) v[a] = profile_u(s, x, tt);
+                if (kd == KIND_OUTLET)
+                    for (int k = 0; k < D; ++k)
+                        if (k != a) v[k] = 0.0;
+            }
+        }
+        for (int k = 0; k < D; ++k) {
+            O.x[p * D + k] = x[k];
             O.v[p * D + k] = v[k];
         }
-        const bool rho_new = mode == 1 || shepard_due(s, dyn);
-        O.rho[p] = rho_new ? s.rhon[i] : S.rho[i];
-        O.p[p] = S.p[i];
-        O.c[p] = S.c[i];
-        const double m = S.m[i], gap = S.gap[i];
+        const bool rho_new = mode == 1 || (shepard_due(s, dyn) && src_fluid && !spawn);
+        const double rho = rho_new ? s.rhon[src] : S.rho[src];
+        const double pp = S.p[src];
+        O.rho[p] = rho;
+        O.p[p] = pp;
+        O.c[p] = S.c[src];
+        const double m = S.m[src], gap = S.gap[src];
         O.m[p] = m;
         O.gap[p] = gap;
-        O.id[p] = S.id[i];
+        O.id[p] = id;
+        if (O.kind) O.kind[p] = kd;
         double Tt = 0.0, kap = 0.0;
         if (thermal) {
-            Tt = s.Tn[i];
-            kap = S.kappa[i];
+            Tt = s.Tn[src];
+            kap = S.kappa[src];
             O.T[p] = Tt;
             O.kappa[p] = kap;
-            O.cp[p] = S.cp[i];
+            O.cp[p] = S.cp[src];
         }
-        if (s.prec == ESPH_FP32_BULK)
+        if (s.prec == ESPH_FP32_BULK) {
             store_sorted_rec<float>(O, D, p, v, m, gap, Tt, kap, thermal);
-        else
+            if (kd != KIND_FLUID) {  // buffers never pass the continuity pass
+                float* B = static_cast<float*>(O.B) + 4 * (size_t)p;
+                B[0] = (float)rho;
+                B[1] = (float)(pp / (rho * rho));
+            }
+        } else {
             store_sorted_rec<double>(O, D, p, v, m, gap, Tt, kap, thermal);
+            if (kd != KIND_FLUID) {
+                double* B = static_cast<double*>(O.B) + 4 * (size_t)p;
+                B[0] = rho;
+                B[1] = pp / (rho * rho);
+            }
+        }
     }
 }
 
@@ -728,9 +873,16 @@ __global__ void k_sort_finish(const SceneDev* __restrict__ scenes, int nscenes, 
     if (sc >= nscenes) return;
     const SceneDev& s = scenes[sc];
     SceneDyn* d = s.dyn;
+    if (mode == 0 && (s.flags & FLAG_IO)) {
+        const int room = s.cap - d->n_live;  // n_live is still the step's input count here
+        d->next_fluid_id += d->nspawn < room ? d->nspawn : room;
+        d->next_outlet_id += d->nleave;
+    }
     d->n_live = s.cell_start[s.ncell];
     d->parity ^= 1;
     d->novf = 0;
+    d->nspawn = 0;
+    d->nleave = 0;
     if (mode == 0) d->step += 1;
 }
 
@@ -751,16 +903,22 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const SceneDev* __restrict__ s
         const int i = tile.begin + (int)threadIdx.x;
         if (i >= dyn.n_live) continue;
         const FluidSet& S = s.set[dyn.parity];
+        const int8_t* kind = S.kind;  // inlet/outlet solver: fluid particles only
+        int64_t w = offs ? offs[i] : 0;
+        int cnt = 0;
+        if (kind && kind[i] != KIND_FLUID) {
+            if (counts) counts[i] = 0;
+            continue;
+        }
         double xi[3];
         load_x<D>(S.x, i, xi);
         int c[3];
         cell_of<D>(s, xi, c);
         const int idi = S.id[i];
-        int64_t w = offs ? offs[i] : 0;
-        int cnt = 0;
         if (pairset == ESPH_PAIRS_FLUID_FLUID) {
             for_each_neighbour<D>(s, s.cell_start, S.x, xi, c, i, [&](int j, double, const double*) {
                 const int idj = S.id[j];
+                if (kind && kind[j] != KIND_FLUID) return;
                 if (idj > idi) {
                     if (out) { out[2 * w] = idi; out[2 * w + 1] = idj; ++w; }
                     ++cnt;
@@ -869,7 +1027,7 @@ hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st) {
     hipLaunchKernelGGL(k_sort_scan1, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
     hipLaunchKernelGGL(k_sort_scan2, dim3(grid_for(p.nscenes)), blk, 0, st, p.scenes, p.nscenes);
     hipLaunchKernelGGL(k_sort_scan3, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
-    hipLaunchKernelGGL(k_sort_place, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles);
+    hipLaunchKernelGGL(k_sort_place, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_scatter, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_finish, dim3((p.nscenes + 63) / 64), dim3(64), 0, st, p.scenes, p.nscenes, mode);
     return hipGetLastError();

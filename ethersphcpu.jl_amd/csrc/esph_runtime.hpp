@@ -88,6 +88,10 @@ struct esph_scene {
     esph::WallStage wall_stage;
     std::vector<esph::Tile> tile_stage_p, tile_stage_c, tile_stage_o;
     bool walls_set = false;
+    int batch_refs = 0;  // live batches holding this scene
+    // inlet/outlet buffers (host copies, inlet then outlet), set before the fluid
+    int64_t n_inlet = 0, n_outlet = 0;
+    std::vector<double> bx, bv, brho, bp, bc, bm, bgap;
     bool allocated = false;
     bool fluid_ready = false;
     int64_t n_upload = 0;

@@ -37,11 +37,36 @@ struct WallCopy {
     std::vector<double> x, normal, gap, v, t, kappa;
 };
 
+struct FluidCopy {
+    int64_t n = 0;
+    std::vector<double> x, v, rho, p, c, m, gap, t, kappa, cp;
+
+    void take(const esph_fluid_arrays& f, int D) {
+        auto cp_ = [](std::vector<double>& dst, const double* src, int64_t k) {
+            if (src && k > 0) dst.assign(src, src + k);
+        };
+        n = f.n;
+        cp_(x, f.x, n * D); cp_(v, f.v, n * D); cp_(rho, f.rho, n); cp_(p, f.p, n);
+        cp_(c, f.c, n); cp_(m, f.mass, n); cp_(gap, f.gap, n);
+        cp_(t, f.t, n); cp_(kappa, f.kappa, n); cp_(cp, f.cp, n);
+    }
+    esph_fluid_arrays view() const {
+        esph_fluid_arrays f{};
+        f.n = n;
+        f.x = x.data(); f.v = v.data(); f.rho = rho.data(); f.p = p.data(); f.c = c.data();
+        f.mass = m.data(); f.gap = gap.data();
+        f.t = t.empty() ? nullptr : t.data();
+        f.kappa = kappa.empty() ? nullptr : kappa.data();
+        f.cp = cp.empty() ? nullptr : cp.data();
+        return f;
+    }
+};
+
 struct Request {
     int64_t id = 0;
     esph_scene_desc desc{};
-    int64_t n = 0;
-    std::vector<double> x, v, rho, p, c, m, gap, t, kappa, cp;
+    FluidCopy fluid, inlet, outlet;
+    bool buffers = false;
     std::vector<WallCopy> walls;
     int64_t nsteps = 0;
     int64_t done = 0;
@@ -102,14 +127,12 @@ static int admit(esph_server* sv) {
             wa.push_back(a);
         }
         if (!rc) rc = esph_scene_set_walls(s, (int32_t)wa.size(), wa.data());
+        if (!rc && r.buffers) {
+            const esph_fluid_arrays in = r.inlet.view(), out = r.outlet.view();
+            rc = esph_scene_set_buffers(s, &in, &out);
+        }
         if (!rc) {
-            esph_fluid_arrays f{};
-            f.n = r.n;
-            f.x = r.x.data(); f.v = r.v.data(); f.rho = r.rho.data(); f.p = r.p.data(); f.c = r.c.data();
-            f.mass = r.m.data(); f.gap = r.gap.data();
-            f.t = r.t.empty() ? nullptr : r.t.data();
-            f.kappa = r.kappa.empty() ? nullptr : r.kappa.data();
-            f.cp = r.cp.empty() ? nullptr : r.cp.data();
+            const esph_fluid_arrays f = r.fluid.view();
             rc = esph_scene_set_fluid(s, &f);
         }
         if (rc == ESPH_ERR_BUDGET) {
@@ -164,8 +187,10 @@ static int retire(esph_server* sv, Request& r) {
     r.state = rc ? ESPH_REQ_FAILED : ESPH_REQ_DONE;
     if (rc) sv->failed++; else sv->completed++;
     // the particle copies are no longer needed
-    r.x.clear(); r.v.clear(); r.rho.clear(); r.p.clear(); r.c.clear(); r.m.clear(); r.gap.clear();
-    r.t.clear(); r.kappa.clear(); r.cp.clear(); r.walls.clear();
+    r.fluid = FluidCopy();
+    r.inlet = FluidCopy();
+    r.outlet = FluidCopy();
+    r.walls.clear();
     return ESPH_OK;
 }
 
@@ -197,24 +222,29 @@ extern "C" int esph_server_destroy(esph_server* sv) {
 }
 
 extern "C" int esph_server_submit(esph_server* sv, const esph_scene_desc* desc, int32_t nwall_groups,
-                                  const esph_wall_arrays* walls, const esph_fluid_arrays* fluid, int64_t nsteps,
+                                  const esph_wall_arrays* walls, const esph_fluid_arrays* fluid,
+                                  const esph_fluid_arrays* inlet, const esph_fluid_arrays* outlet, int64_t nsteps,
                                   int64_t* request_id) {
-    if (!sv || !desc || !fluid || nsteps < 0 || nwall_groups < 0 || (nwall_groups > 0 && !walls))
+    if (!sv || !desc || !fluid || nsteps < 0 || nwall_groups < 0 || (nwall_groups > 0 && !walls) ||
+        (!inlet) != (!outlet))
         return fail(sv ? sv->ctx : nullptr, ESPH_ERR_ARG, "server_submit: bad argument");
     const int D = desc->dim;
     if (D != 2 && D != 3) return fail(sv->ctx, ESPH_ERR_ARG, "server_submit: dim must be 2 or 3");
+    if ((inlet != nullptr) != (desc->solver == ESPH_SOLVER_FLUID_INLET_OUTLET_WALL))
+        return fail(sv->ctx, ESPH_ERR_ARG, "server_submit: buffers go with the inlet/outlet solver and only with it");
     auto r = std::make_unique<Request>();
     r->id = sv->next_id++;
     r->desc = *desc;
-    r->n = fluid->n;
     r->nsteps = nsteps;
+    r->fluid.take(*fluid, D);
+    if (inlet) {
+        r->buffers = true;
+        r->inlet.take(*inlet, D);
+        r->outlet.take(*outlet, D);
+    }
     auto cp = [](std::vector<double>& dst, const double* src, int64_t n) {
         if (src && n > 0) dst.assign(src, src + n);
     };
-    const int64_t n = fluid->n;
-    cp(r->x, fluid->x, n * D); cp(r->v, fluid->v, n * D); cp(r->rho, fluid->rho, n); cp(r->p, fluid->p, n);
-    cp(r->c, fluid->c, n); cp(r->m, fluid->mass, n); cp(r->gap, fluid->gap, n);
-    cp(r->t, fluid->t, n); cp(r->kappa, fluid->kappa, n); cp(r->cp, fluid->cp, n);
     for (int q = 0; q < nwall_groups; ++q) {
         WallCopy w;
         w.kind = walls[q].kind;

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_uint64, c_void_p
 
 import numpy as np
@@ -23,7 +24,9 @@ ERRORS = {1: "ESPH_ERR_ARG", 2: "ESPH_ERR_HIP", 3: "ESPH_ERR_BUDGET", 4: "ESPH_E
 
 CUBIC_SPLINE, GAUSSIAN, WENDLAND_C2, WENDLAND_C4 = 0, 1, 2, 3
 SOLVER_FLUID_WALL, SOLVER_FLUID_VELOCITY, SOLVER_FLUID_THERMOSTATIC_WALL, \
-    SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL = 0, 1, 2, 3
+    SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL, SOLVER_FLUID_INLET_OUTLET_WALL = 0, 1, 2, 3, 4
+PROFILE_NONE, PROFILE_CHANNEL_2D, PROFILE_PIPE_3D = 0, 1, 2
+GROUP_FLUID, GROUP_INLET, GROUP_OUTLET = 0, 1, 2
 FP64, FP32_BULK = 0, 1
 WALL_COMPULSIVE, WALL_STATIC_VELOCITY, WALL_THERMOSTATIC = 0, 1, 2
 FIELD_X, FIELD_V, FIELD_RHO, FIELD_P, FIELD_C, FIELD_MASS, FIELD_GAP, FIELD_T, FIELD_KAPPA, \
@@ -44,6 +47,9 @@ class SceneDesc(ctypes.Structure):
         ("dt", c_double), ("density_reinitialized_interval", c_int64), ("check_interval", c_int64),
         ("check_enabled", c_int32), ("check_min", D3), ("check_max", D3),
         ("fluid_capacity", c_int64), ("domain_set", c_int32), ("domain_min", D3), ("domain_max", D3),
+        ("io_axis", c_int32), ("inlet_enter", c_double), ("inlet_shift", c_double), ("outlet_enter", c_double),
+        ("outlet_exit", c_double), ("profile", c_int32), ("profile_terms", c_int32), ("outlet_profile", c_int32),
+        ("profile_f", c_double), ("profile_nu", c_double), ("profile_l", c_double), ("profile_center", D3),
     ]
 
 
@@ -91,6 +97,9 @@ def lib() -> ctypes.CDLL:
         "esph_scene_destroy": ([vp], c_int32),
         "esph_scene_set_walls": ([vp, c_int32, POINTER(WallArrays)], c_int32),
         "esph_scene_set_fluid": ([vp, POINTER(FluidArrays)], c_int32),
+        "esph_scene_set_buffers": ([vp, POINTER(FluidArrays), POINTER(FluidArrays)], c_int32),
+        "esph_scene_group_count": ([vp, c_int32, POINTER(c_int64)], c_int32),
+        "esph_scene_get_group": ([vp, c_int32, c_int32, DP, c_int64, POINTER(c_int64)], c_int32),
         "esph_scene_step": ([vp, c_int64], c_int32),
         "esph_scene_sync": ([vp], c_int32),
         "esph_scene_step_count": ([vp, POINTER(c_int64)], c_int32),
@@ -108,8 +117,8 @@ def lib() -> ctypes.CDLL:
         "esph_batch_sync": ([vp], c_int32),
         "esph_server_create": ([vp, c_int32, POINTER(vp)], c_int32),
         "esph_server_destroy": ([vp], c_int32),
-        "esph_server_submit": ([vp, POINTER(SceneDesc), c_int32, POINTER(WallArrays), POINTER(FluidArrays), c_int64,
-                                POINTER(c_int64)], c_int32),
+        "esph_server_submit": ([vp, POINTER(SceneDesc), c_int32, POINTER(WallArrays), POINTER(FluidArrays),
+                                POINTER(FluidArrays), POINTER(FluidArrays), c_int64, POINTER(c_int64)], c_int32),
         "esph_server_advance": ([vp, c_int64, POINTER(c_int64)], c_int32),
         "esph_server_status": ([vp, c_int64, POINTER(c_int32), POINTER(c_int64)], c_int32),
         "esph_server_result": ([vp, c_int64, c_int32, DP, c_int64, POINTER(c_int64)], c_int32),
@@ -146,6 +155,11 @@ class Context:
         if rc != ESPH_OK:
             raise EsphError(rc, "esph_ctx_create failed (no GPU?)")
         self.handle = h
+        # scenes, batches and servers of this context, closed before it
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj) -> None:
+        self._children.add(obj)
 
     def check(self, rc: int) -> None:
         if rc != ESPH_OK:
@@ -158,6 +172,12 @@ class Context:
 
     def close(self) -> None:
         if self.handle:
+            # servers and batches first: they reference scenes
+            kids = list(getattr(self, "_children", ()))
+            for rank in (0, 1, 2):
+                for k in kids:
+                    if getattr(k, "_close_rank", 2) == rank:
+                        k.close()
             self._L.esph_ctx_destroy(self.handle)
             self.handle = None
 

@@ -12,6 +12,12 @@ wall layout, check predicate), with the sizes of the workload it names:
   box 3.2 x 1.0 x 1.6 m (z up), water 1.2 x 1.0 x 0.6 m, dx = 0.0089 ->
   1,013,040 fluid, 645,804 wall (five faces, no lid), WendlandC4, h = 1.3 dx.
 - ``lid_driven_cavity_2d`` (examples/LidDrivenCavity/LidDrivenCavity.jl:10-156).
+- ``poiseuille_2d`` channel with inlet/outlet buffers
+  (examples/PoiseuilleFlow2D/PoiseuilleFlow2D.jl:11-169): l = 1e-2, dr = l / 50
+  -> 10,000 fluid, 150 + 150 buffer, 1,272 wall; WendlandC2, R = 3 dr.
+- ``poiseuille_3d`` pipe with inlet/outlet buffers
+  (examples/PoiseuilleFlow3D/PoiseuilleFlow3D.jl:11-158): cylinder-ring
+  lattice, dr = l / 40.
 
 Lattice coordinates are computed with the reference's operation order,
 x = (x0 + (j - 1) dx) + dx / 2, so that lattice pairs at exactly the cutoff
@@ -27,7 +33,7 @@ import numpy as np
 from . import _lib
 from .models import CommonWeaklyCompressibleLiquidModel, ThermalWeaklyCompressibleLiquidModel
 from .particles import LiquidParticles, WallParticles
-from .solve import AABBCheck, Scene, make_desc, select_solver
+from .solve import AABBCheck, BufferProfile, InletOutletPlanes, Scene, make_desc, select_solver
 from .sph_kernel import KERNELS, CubicSpline, SmoothKernel, WendlandC2, WendlandC4
 from .time_discretization import DensityReinitializedFixedTimeIntervalForwardEuler
 
@@ -42,10 +48,13 @@ class Scenario:
     fluid: LiquidParticles
     walls: list = field(default_factory=list)
     check: AABBCheck | None = None
+    buffers: tuple | None = None  # (inlet, outlet) of the inlet/outlet solver
+    planes: InletOutletPlanes | None = None
+    profile: BufferProfile | None = None
 
     @property
     def solver(self) -> int:
-        return select_solver(self.walls, self.model)
+        return select_solver(self.walls, self.model, buffers=self.buffers is not None)
 
     @property
     def n_fluid(self) -> int:
@@ -55,11 +64,17 @@ class Scenario:
     def n_wall(self) -> int:
         return sum(len(w) for w in self.walls)
 
+    @property
+    def n_particles(self) -> int:
+        """Fluid plus buffer particles (the device particle arrays)."""
+        return self.n_fluid + (sum(len(b) for b in self.buffers) if self.buffers else 0)
+
     def desc(self, precision: int = _lib.FP64, capacity: int = 0) -> _lib.SceneDesc:
-        return make_desc(self.dim, self.kernel, self.model, self.td, self.solver, precision, self.check, capacity)
+        return make_desc(self.dim, self.kernel, self.model, self.td, self.solver, precision, self.check, capacity,
+                         planes=self.planes, profile=self.profile)
 
     def make_scene(self, precision: int = _lib.FP64, ctx=None, capacity: int = 0) -> Scene:
-        return Scene(self.desc(precision, capacity), self.fluid, self.walls, ctx)
+        return Scene(self.desc(precision, capacity), self.fluid, self.walls, ctx, buffers=self.buffers)
 
 
 def rectangle(dx, x0, y0, width_number, height_number):
@@ -291,3 +306,151 @@ def perturb(sc: Scenario, amplitude: float, seed: int = 0) -> Scenario:
     f.x_vec_ = f.x_vec_ + rng.uniform(-amplitude, amplitude, f.x_vec_.shape) * dx
     f.v_vec_ = f.v_vec_ + rng.uniform(-0.1, 0.1, f.v_vec_.shape)
     return sc
+
+
+def _channel_ux(y, t, f, nu, l, terms):
+    """PoiseuilleFlow2D.jl:28-35 (y measured from the bottom wall)."""
+    ux = f / 2.0 / nu * y * (l - y)
+    for n in range(terms + 1):
+        o = float(2 * n + 1)
+        ux = ux - 4.0 * f * (l * l) / nu / (math.pi ** 3) / (o * o * o) * np.sin(math.pi * y / l * o) * np.exp(
+            -(o * o) * (math.pi * math.pi) * nu * t / (l * l))
+    return ux
+
+
+def poiseuille_2d(rows=50, reynolds=None, l=1e-2, nu=1e-6, rho_0=1000.0, f=8e-5, kernel=WendlandC2,
+                  radius_factor=3.0, length_factor=4, terms=10, total_step=1000, reinit=5, check=5):
+    """2D Poiseuille channel with inlet and outlet buffers
+    (PoiseuilleFlow2D.jl:11-169): ``rows`` lattice rows across the channel
+    (dr = l / rows; the example has 50), ``length_factor * l`` long, three
+    buffer columns at each end, three compulsive-wall layers above and below
+    that reach three columns past the buffers.  The fluid starts on the
+    analytic start-up profile at t = 0.  ``reynolds`` (Re = u_max l / nu),
+    when given, replaces the body force: f = 8 nu^2 Re / l^3."""
+    dim = 2
+    dr = l / rows
+    gap = dr
+    if reynolds is not None:
+        f = 8.0 * nu * nu * reynolds / (l * l * l)
+    R = radius_factor * dr
+    sk = SmoothKernel(R, dim, kernel if not isinstance(kernel, str) else KERNELS[kernel])
+    ux_max = f / 2 / nu / 4 * l ** 2
+    c_0 = 10 * ux_max
+    p_0 = 0.02 * rho_0 * c_0 ** 2
+    mu_0 = nu * rho_0
+    gamma = 7
+    model = CommonWeaklyCompressibleLiquidModel(rho_0, c_0, p_0, gamma, mu_0, [f, 0.0])
+    dt = 0.1 * sk.influence_radius_ / c_0
+    td = DensityReinitializedFixedTimeIntervalForwardEuler(dt, total_step, 100, check, reinit)
+    x0 = y0 = 0.0
+    fluid_width = l * length_factor
+    n_row = round(l / dr)
+    n_col = round(fluid_width / dr)
+
+    def liquid(X):
+        p = LiquidParticles(dim, len(X))
+        p.x_vec_ = X
+        p.p_ = np.full(len(p), p_0)
+        p.rho_ = np.full(len(p), rho_0)
+        p.c_ = np.full(len(p), c_0)
+        p.mass_ = np.full(len(p), dr ** 2 * rho_0)
+        p.gap_ = np.full(len(p), gap)
+        p.v_vec_[:, 0] = _channel_ux(X[:, 1] - y0, 0.0, f, nu, l, terms)
+        return p
+
+    t = 3
+    fl = liquid(rectangle(dr, x0, y0, n_col, n_row))
+    inlet = liquid(rectangle(dr, x0 - t * dr, y0, t, n_row))
+    outlet = liquid(rectangle(dr, x0 + fluid_width, y0, t, n_row))
+    n_more = 3
+    ww = round(fluid_width / dr) + 2 * (t + n_more)
+    C = _lib.WALL_COMPULSIVE
+    groups = [
+        _walls(C, rectangle(dr, x0 - (t + n_more) * dr, y0 - t * dr, ww, t), [0.0, 1.0], gap),
+        _walls(C, rectangle(dr, x0 - (t + n_more) * dr, y0 + l, ww, t), [0.0, -1.0], gap),
+    ]
+    walls = WallParticles.concat(groups)
+    planes = InletOutletPlanes(0, x0, t * dr, x0 + fluid_width, x0 + fluid_width + t * dr)
+    profile = BufferProfile(_lib.PROFILE_CHANNEL_2D, f, nu, l, (x0, y0, 0.0), terms, outlet_profile=True)
+    return Scenario("poiseuille_2d", dim, sk, model, td, fl, [walls], None, (inlet, outlet), planes, profile)
+
+
+def cylinder_ring(dx, x_0, y_0, z_0, inner_r, outer_r, length, rho=None):
+    """createCylinderRingParticles (PoiseuilleFlow3D.jl:60-93): rings of
+    particles along x; returns positions, masses (when rho is given) and gaps."""
+    len_num = round(length / dx)
+    len_dx = length / len_num
+    r_num = round((outer_r - inner_r) / dx)
+    r_dx = (outer_r - inner_r) / r_num
+    X, M, G = [], [], []
+    for i_len in range(1, len_num + 1):
+        x_here = x_0 + (i_len - 0.5) * len_dx
+        for i_r in range(1, r_num + 1):
+            r_here = inner_r + (i_r - 0.5) * r_dx
+            n_theta = round(2 * math.pi * r_here / r_dx)
+            if n_theta == 0:
+                n_theta = 1
+            d_theta = 2 * math.pi / n_theta
+            dr_here = 2 * math.pi * r_here / n_theta
+            th = np.arange(1, n_theta + 1) * d_theta
+            blk = np.empty((n_theta, 3))
+            blk[:, 0] = x_here
+            blk[:, 1] = y_0 + r_here * np.cos(th)
+            blk[:, 2] = z_0 + r_here * np.sin(th)
+            X.append(blk)
+            M.append(np.full(n_theta, len_dx * dr_here * r_dx * rho if rho is not None else 0.0))
+            G.append(np.full(n_theta, (len_dx + dr_here + r_dx) / 3))
+    return np.concatenate(X), np.concatenate(M), np.concatenate(G)
+
+
+def poiseuille_3d(cells=40, l=1e-2, nu=1e-6, rho_0=1000.0, f=8e-5, kernel=WendlandC2, radius_factor=3.0,
+                  length_factor=2, total_step=1000, reinit=5, check=5):
+    """3D Poiseuille pipe with inlet and outlet buffers
+    (PoiseuilleFlow3D.jl:11-158): radius l / 2, dr = l / ``cells`` (the
+    example has 40), ``length_factor * l`` long, three buffer rings at each
+    end and three compulsive-wall rings around a pipe 12 dr longer.  The
+    fluid starts on the steady profile; the outlet keeps its axial velocity
+    (only its cross components are zeroed after each step)."""
+    dim = 3
+    r0 = l / 2
+    dr = l / cells
+    R = radius_factor * dr
+    sk = SmoothKernel(R, dim, kernel if not isinstance(kernel, str) else KERNELS[kernel])
+    ux_max = r0 ** 2 / 4 * f / nu
+    c_0 = 10 * ux_max
+    p_0 = 0.05 * rho_0 * c_0 ** 2
+    mu_0 = nu * rho_0
+    gamma = 7
+    model = CommonWeaklyCompressibleLiquidModel(rho_0, c_0, p_0, gamma, mu_0, [f, 0.0, 0.0])
+    dt = 0.1 * sk.influence_radius_ / c_0
+    td = DensityReinitializedFixedTimeIntervalForwardEuler(dt, total_step, 10, check, reinit)
+    x0 = y0 = z0 = 0.0
+    pipe_len = length_factor * l
+
+    def liquid(X, M, G):
+        p = LiquidParticles(dim, len(X))
+        p.x_vec_ = X
+        p.mass_ = M
+        p.gap_ = G
+        p.p_ = np.full(len(p), p_0)
+        p.rho_ = np.full(len(p), rho_0)
+        p.c_ = np.full(len(p), c_0)
+        p.v_vec_[:, 0] = f / 4 / nu * (r0 ** 2 - X[:, 1] ** 2 - X[:, 2] ** 2)
+        return p
+
+    t = 3
+    fl = liquid(*cylinder_ring(dr, x0, y0, z0, 0.0, r0, pipe_len, rho_0))
+    inlet = liquid(*cylinder_ring(dr, x0 - t * dr, y0, z0, 0.0, r0, t * dr, rho_0))
+    outlet = liquid(*cylinder_ring(dr, x0 + pipe_len, y0, z0, 0.0, r0, t * dr, rho_0))
+    WX, _, WG = cylinder_ring(dr, x0 - 2 * t * dr, y0, z0, r0, r0 + t * dr, pipe_len + 4 * t * dr)
+    nrm = np.zeros_like(WX)
+    rr = np.sqrt(WX[:, 1] ** 2 + WX[:, 2] ** 2)
+    nrm[:, 1] = -WX[:, 1] / rr
+    nrm[:, 2] = -WX[:, 2] / rr
+    walls = WallParticles(_lib.WALL_COMPULSIVE, dim, len(WX))
+    walls.x_vec_ = WX
+    walls.normal_vec_ = nrm
+    walls.gap_ = WG
+    planes = InletOutletPlanes(0, x0, t * dr, x0 + pipe_len, x0 + pipe_len + t * dr)
+    profile = BufferProfile(_lib.PROFILE_PIPE_3D, f, nu, r0, (x0, y0, z0), 0, outlet_profile=False)
+    return Scenario("poiseuille_3d", dim, sk, model, td, fl, [walls], None, (inlet, outlet), planes, profile)

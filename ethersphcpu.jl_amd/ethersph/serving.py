@@ -13,19 +13,25 @@ from ctypes import byref, c_double, c_int32, c_int64, c_void_p
 import numpy as np
 
 from . import _lib
+from .solve import fluid_arrays
 
 
 class Server:
+    _close_rank = 0
+
     def __init__(self, ctx: _lib.Context, max_active: int = 0):
         self.ctx = ctx
         self.L = ctx._L
         h = c_void_p()
         ctx.check(self.L.esph_server_create(ctx.handle, max_active, byref(h)))
         self.handle = h
+        ctx.adopt(self)
         self.meta = {}
 
-    def submit(self, desc: _lib.SceneDesc, fluid, walls, nsteps: int) -> int:
-        """Queues a request; fluid is a LiquidParticles, walls a list of WallParticles."""
+    def submit(self, desc: _lib.SceneDesc, fluid, walls, nsteps: int, buffers: tuple | None = None) -> int:
+        """Queues a request; fluid is a LiquidParticles, walls a list of
+        WallParticles, buffers the (inlet, outlet) LiquidParticles of the
+        inlet/outlet solver."""
         keep = []
         warr = (_lib.WallArrays * max(len(walls), 1))()
         for q, w in enumerate(walls):
@@ -33,21 +39,23 @@ class Server:
             keep += a
             warr[q].kind, warr[q].n = w.kind, len(w)
             warr[q].x, warr[q].normal, warr[q].gap, warr[q].v, warr[q].t, warr[q].kappa = (_lib.dptr(v) for v in a)
-        fa = _lib.FluidArrays()
         thermal = getattr(fluid, "thermal", False)
-        names = ("x_vec_", "v_vec_", "rho_", "p_", "c_", "mass_", "gap_") + (("t_", "kappa_", "cp_") if thermal else ())
-        arrs = {k: _lib.f64(getattr(fluid, k)) for k in names}
-        keep += list(arrs.values())
-        fa.n = len(fluid)
-        fa.x, fa.v, fa.rho, fa.p, fa.c = (_lib.dptr(arrs[k]) for k in ("x_vec_", "v_vec_", "rho_", "p_", "c_"))
-        fa.mass, fa.gap = _lib.dptr(arrs["mass_"]), _lib.dptr(arrs["gap_"])
-        if thermal:
-            fa.t, fa.kappa, fa.cp = (_lib.dptr(arrs[k]) for k in ("t_", "kappa_", "cp_"))
+        fa, k0 = fluid_arrays(fluid, thermal)
+        ia = oa = None
+        if buffers is not None:
+            ia, k1 = fluid_arrays(buffers[0], False)
+            oa, k2 = fluid_arrays(buffers[1], False)
+            keep += [k1, k2]
         rid = c_int64()
-        self.ctx.check(self.L.esph_server_submit(self.handle, byref(desc), len(walls), warr, byref(fa), nsteps,
-                                                 byref(rid)))
+        self.ctx.check(self.L.esph_server_submit(self.handle, byref(desc), len(walls), warr, byref(fa),
+                                                 byref(ia) if ia is not None else None,
+                                                 byref(oa) if oa is not None else None, nsteps, byref(rid)))
         self.meta[rid.value] = (desc.dim, thermal)
         return rid.value
+
+    def submit_scenario(self, sc, precision: int, nsteps: int) -> int:
+        """Queues a scenes.Scenario (its descriptor at the given precision)."""
+        return self.submit(sc.desc(precision), sc.fluid, sc.walls, nsteps, getattr(sc, "buffers", None))
 
     def advance(self, max_steps: int) -> int:
         k = c_int64()

@@ -29,6 +29,7 @@ SOLVER_NAMES = {
     "FluidVelocity": _lib.SOLVER_FLUID_VELOCITY,
     "FluidThermostaticWall": _lib.SOLVER_FLUID_THERMOSTATIC_WALL,
     "FluidCompulsiveThermostaticWall": _lib.SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL,
+    "FluidInletOutletWall": _lib.SOLVER_FLUID_INLET_OUTLET_WALL,
 }
 PRECISIONS = {"FP64": _lib.FP64, "FP32_BULK": _lib.FP32_BULK}
 
@@ -52,10 +53,51 @@ class AABBCheck:
         return lo, hi
 
 
-def select_solver(walls: list, model) -> int:
+class InletOutletPlanes:
+    """The inlet/outlet closures of FluidInletOutletWallSolve.jl:176-180 as
+    data, in the form the Poiseuille examples use (PoiseuilleFlow2D.jl:146-151):
+    along ``axis``, an inlet particle past ``inlet_enter`` is copied into the
+    fluid and moved back by ``inlet_shift``; a fluid particle past
+    ``outlet_enter`` moves to the outlet; an outlet particle past
+    ``outlet_exit`` is removed."""
+
+    def __init__(self, axis: int, inlet_enter: float, inlet_shift: float, outlet_enter: float,
+                 outlet_exit: float):
+        self.axis = int(axis)
+        self.inlet_enter = float(inlet_enter)
+        self.inlet_shift = float(inlet_shift)
+        self.outlet_enter = float(outlet_enter)
+        self.outlet_exit = float(outlet_exit)
+
+
+class BufferProfile:
+    """The examples' user_defined_function as data: buffer velocities along
+    the axis set after every step at t = step * dt.
+
+    kind CHANNEL_2D: the start-up channel series of PoiseuilleFlow2D.jl:28-35
+    (``terms`` is the last n of the sum); PIPE_3D: the steady pipe profile of
+    PoiseuilleFlow3D.jl:32-34.  ``outlet_profile`` sets the outlet's axial
+    velocity too (2D example); the outlet's cross components are zeroed."""
+
+    def __init__(self, kind: int, f: float, nu: float, l: float, center=(0.0, 0.0, 0.0), terms: int = 10,
+                 outlet_profile: bool = True):
+        self.kind = int(kind)
+        self.f = float(f)
+        self.nu = float(nu)
+        self.l = float(l)
+        self.center = tuple(float(c) for c in center) + (0.0,) * (3 - len(center))
+        self.terms = int(terms)
+        self.outlet_profile = bool(outlet_profile)
+
+
+def select_solver(walls: list, model, buffers: bool = False) -> int:
     """Dispatch of solve! on particle-vector and model types."""
     thermal = isinstance(model, ThermalWeaklyCompressibleLiquidModel)
     kinds = [w.kind for w in walls]
+    if buffers:  # FluidInletOutletWallSolve.jl:167-181
+        if len(walls) == 1 and not thermal:
+            return _lib.SOLVER_FLUID_INLET_OUTLET_WALL
+        raise TypeError("the inlet/outlet solver takes one wall group and a common liquid model")
     if len(walls) == 2 and thermal and kinds[1] == _lib.WALL_THERMOSTATIC:
         return _lib.SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL
     if len(walls) == 1:
@@ -70,7 +112,8 @@ def select_solver(walls: list, model) -> int:
 
 
 def make_desc(dim, kernel: SmoothKernel_, model, td, solver: int, precision: int = _lib.FP64,
-              check: AABBCheck | None = None, capacity: int = 0, domain=None) -> _lib.SceneDesc:
+              check: AABBCheck | None = None, capacity: int = 0, domain=None,
+              planes: InletOutletPlanes | None = None, profile: BufferProfile | None = None) -> _lib.SceneDesc:
     d = _lib.SceneDesc()
     d.dim = dim
     d.kernel = kernel.kind
@@ -107,14 +150,31 @@ def make_desc(dim, kernel: SmoothKernel_, model, td, solver: int, precision: int
         hi[:dim] = domain[1]
         d.domain_min = _lib.D3(*lo)
         d.domain_max = _lib.D3(*hi)
+    if planes is not None:
+        d.io_axis = planes.axis
+        d.inlet_enter = planes.inlet_enter
+        d.inlet_shift = planes.inlet_shift
+        d.outlet_enter = planes.outlet_enter
+        d.outlet_exit = planes.outlet_exit
+    if profile is not None:
+        d.profile = profile.kind
+        d.profile_terms = profile.terms
+        d.outlet_profile = int(profile.outlet_profile)
+        d.profile_f = profile.f
+        d.profile_nu = profile.nu
+        d.profile_l = profile.l
+        d.profile_center = _lib.D3(*profile.center)
     return d
 
 
 class Scene:
     """A scene resident on the device (esph_scene)."""
 
+    _close_rank = 2
+
     def __init__(self, desc: _lib.SceneDesc, fluid: LiquidParticles, walls: list[WallParticles],
-                 ctx: _lib.Context | None = None):
+                 ctx: _lib.Context | None = None, buffers: tuple | None = None):
+        """buffers: (inlet, outlet) LiquidParticles of the inlet/outlet solver."""
         self.ctx = ctx or _lib.default_context()
         self.L = self.ctx._L
         self.dim = desc.dim
@@ -124,8 +184,11 @@ class Scene:
         h = c_void_p()
         self.ctx.check(self.L.esph_scene_create(self.ctx.handle, byref(desc), byref(h)))
         self.handle = h
+        self.ctx.adopt(self)
         try:
             self._set_walls(walls)
+            if buffers is not None:
+                self.set_buffers(*buffers)
             self.set_fluid(fluid)
         except Exception:
             self.close()
@@ -150,18 +213,14 @@ class Scene:
         self.ctx.check(self.L.esph_scene_set_walls(self.handle, len(walls), arr))
 
     def set_fluid(self, f: LiquidParticles) -> None:
-        a = _lib.FluidArrays()
-        keep = {k: _lib.f64(getattr(f, k)) for k in ("x_vec_", "v_vec_", "rho_", "p_", "c_", "mass_", "gap_")}
-        if self.thermal:
-            for k in ("t_", "kappa_", "cp_"):
-                keep[k] = _lib.f64(getattr(f, k))
-        a.n = len(f)
-        a.x, a.v = _lib.dptr(keep["x_vec_"]), _lib.dptr(keep["v_vec_"])
-        a.rho, a.p, a.c = _lib.dptr(keep["rho_"]), _lib.dptr(keep["p_"]), _lib.dptr(keep["c_"])
-        a.mass, a.gap = _lib.dptr(keep["mass_"]), _lib.dptr(keep["gap_"])
-        if self.thermal:
-            a.t, a.kappa, a.cp = _lib.dptr(keep["t_"]), _lib.dptr(keep["kappa_"]), _lib.dptr(keep["cp_"])
+        a, keep = fluid_arrays(f, self.thermal)
         self.ctx.check(self.L.esph_scene_set_fluid(self.handle, byref(a)))
+
+    def set_buffers(self, inlet: LiquidParticles, outlet: LiquidParticles) -> None:
+        """Inlet and outlet buffer particles (FluidInletOutletWallSolve.jl:8-20)."""
+        a, k1 = fluid_arrays(inlet, False)
+        b, k2 = fluid_arrays(outlet, False)
+        self.ctx.check(self.L.esph_scene_set_buffers(self.handle, byref(a), byref(b)))
 
     def step(self, nsteps: int = 1) -> None:
         self.ctx.check(self.L.esph_scene_step(self.handle, nsteps))
@@ -180,26 +239,35 @@ class Scene:
         self.ctx.check(self.L.esph_scene_fluid_count(self.handle, byref(v)))
         return int(v.value)
 
-    def get(self, field: int) -> np.ndarray:
-        n = self.fluid_count()
+    def group_count(self, group: int) -> int:
+        v = c_int64()
+        self.ctx.check(self.L.esph_scene_group_count(self.handle, group, byref(v)))
+        return int(v.value)
+
+    def get(self, field: int, group: int = _lib.GROUP_FLUID) -> np.ndarray:
+        n = self.group_count(group)
         w = self.dim if field in (_lib.FIELD_X, _lib.FIELD_V) else 1
         out = np.empty(max(n * w, 1))
         wr = c_int64()
-        self.ctx.check(self.L.esph_scene_get_fluid(self.handle, field, _lib.dptr(out), out.size, byref(wr)))
+        if group == _lib.GROUP_FLUID:
+            self.ctx.check(self.L.esph_scene_get_fluid(self.handle, field, _lib.dptr(out), out.size, byref(wr)))
+        else:
+            self.ctx.check(self.L.esph_scene_get_group(self.handle, group, field, _lib.dptr(out), out.size,
+                                                       byref(wr)))
         out = out[: wr.value]
         return out.reshape(n, w) if w > 1 else out
 
-    def state(self) -> LiquidParticles:
-        n = self.fluid_count()
+    def state(self, group: int = _lib.GROUP_FLUID) -> LiquidParticles:
+        n = self.group_count(group)
         f = LiquidParticles(self.dim, n, self.thermal)
-        f.x_vec_ = self.get(_lib.FIELD_X).reshape(n, self.dim)
-        f.v_vec_ = self.get(_lib.FIELD_V).reshape(n, self.dim)
+        f.x_vec_ = self.get(_lib.FIELD_X, group).reshape(n, self.dim)
+        f.v_vec_ = self.get(_lib.FIELD_V, group).reshape(n, self.dim)
         for name, fid in (("rho_", _lib.FIELD_RHO), ("p_", _lib.FIELD_P), ("c_", _lib.FIELD_C),
                           ("mass_", _lib.FIELD_MASS), ("gap_", _lib.FIELD_GAP)):
-            setattr(f, name, self.get(fid))
+            setattr(f, name, self.get(fid, group))
         if self.thermal:
             for name, fid in (("t_", _lib.FIELD_T), ("kappa_", _lib.FIELD_KAPPA), ("cp_", _lib.FIELD_CP)):
-                setattr(f, name, self.get(fid))
+                setattr(f, name, self.get(fid, group))
         return f
 
     def ids(self) -> np.ndarray:
@@ -236,7 +304,8 @@ class Scene:
 
     def close(self) -> None:
         if getattr(self, "handle", None):
-            self.L.esph_scene_destroy(self.handle)
+            # refused while a live batch holds the scene
+            self.ctx.check(self.L.esph_scene_destroy(self.handle))
             self.handle = None
 
     def __del__(self):
@@ -244,6 +313,23 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+def fluid_arrays(f: LiquidParticles, thermal: bool):
+    """esph_fluid_arrays borrowing f's columns; keep the returned arrays alive
+    for the duration of the call."""
+    a = _lib.FluidArrays()
+    keep = {k: _lib.f64(getattr(f, k)) for k in ("x_vec_", "v_vec_", "rho_", "p_", "c_", "mass_", "gap_")}
+    if thermal:
+        for k in ("t_", "kappa_", "cp_"):
+            keep[k] = _lib.f64(getattr(f, k))
+    a.n = len(f)
+    a.x, a.v = _lib.dptr(keep["x_vec_"]), _lib.dptr(keep["v_vec_"])
+    a.rho, a.p, a.c = _lib.dptr(keep["rho_"]), _lib.dptr(keep["p_"]), _lib.dptr(keep["c_"])
+    a.mass, a.gap = _lib.dptr(keep["mass_"]), _lib.dptr(keep["gap_"])
+    if thermal:
+        a.t, a.kappa, a.cp = _lib.dptr(keep["t_"]), _lib.dptr(keep["kappa_"]), _lib.dptr(keep["cp_"])
+    return a, keep
 
 
 def _as_fluid(fluid):
@@ -258,9 +344,19 @@ def _as_walls(w):
     return WallParticles.from_particles(w)
 
 
-def solve(*args, check=None, precision: str = "FP64", ctx=None, on_output=None):
+def _is_fluid_group(g) -> bool:
+    if isinstance(g, LiquidParticles):
+        return True
+    return isinstance(g, (list, tuple)) and len(g) > 0 and not hasattr(g[0], "normal_vec_")
+
+
+def solve(*args, check=None, precision: str = "FP64", ctx=None, on_output=None, inlet_outlet=None,
+          user_defined_function=None):
     """solve!(fluid_particles, wall_particles..., smooth_kernel, model,
-    time_discretization[, vtp_io]; check).
+    time_discretization[, vtp_io]; check), and the inlet/outlet form
+    solve!(fluid, inlet, outlet, walls, kernel, model, td[, vtp_io];
+    inlet_outlet=InletOutletPlanes, user_defined_function=BufferProfile)
+    (FluidInletOutletWallSolve.jl:167-181).
 
     Particle groups come first; the first SmoothKernel starts the trailing
     (kernel, model, time_discretization[, vtp_io]) arguments.  ``check`` must
@@ -273,12 +369,18 @@ def solve(*args, check=None, precision: str = "FP64", ctx=None, on_output=None):
     kernel, model, td = args[k], args[k + 1], args[k + 2]
     if check is not None and not isinstance(check, AABBCheck):
         raise TypeError("check must be an AABBCheck: device steps cannot call host closures")
+    io = len(groups) == 4 and all(_is_fluid_group(g) for g in groups[:3])
+    if io and inlet_outlet is None:
+        raise TypeError("the inlet/outlet solver needs inlet_outlet=InletOutletPlanes(...)")
     fluid_in = groups[0]
     fluid = _as_fluid(fluid_in)
-    walls = [_as_walls(w) for w in groups[1:]]
-    solver = select_solver(walls, model)
-    desc = make_desc(fluid.dim, kernel, model, td, solver, PRECISIONS[precision], check)
-    scene = Scene(desc, fluid, walls, ctx)
+    bufs_in = groups[1:3] if io else []
+    bufs = [_as_fluid(b) for b in bufs_in]
+    walls = [_as_walls(w) for w in groups[3 if io else 1:]]
+    solver = select_solver(walls, model, buffers=io)
+    desc = make_desc(fluid.dim, kernel, model, td, solver, PRECISIONS[precision], check,
+                     planes=inlet_outlet if io else None, profile=user_defined_function if io else None)
+    scene = Scene(desc, fluid, walls, ctx, buffers=tuple(bufs) if io else None)
     try:
         total = td.total_step_
         done = 0
@@ -293,8 +395,15 @@ def solve(*args, check=None, precision: str = "FP64", ctx=None, on_output=None):
         scene.sync()
         out = scene.state()
         ids = scene.ids()
+        bouts = [scene.state(g) for g in (_lib.GROUP_INLET, _lib.GROUP_OUTLET)] if io else []
     finally:
         scene.close()
+    if io:
+        # the groups change size: fluid gains inlet copies and loses leavers,
+        # the outlet gains them; results replace the callers' contents
+        for dst, st in zip([fluid_in] + list(bufs_in), [out] + bouts):
+            _write_group(dst, st)
+        return None
     if isinstance(fluid_in, LiquidParticles):
         for f in ("x_vec_", "v_vec_") + LiquidParticles.SCALARS + (LiquidParticles.THERMAL if fluid.thermal else ()):
             setattr(fluid_in, f, getattr(out, f))
@@ -303,6 +412,20 @@ def solve(*args, check=None, precision: str = "FP64", ctx=None, on_output=None):
         fluid_in[:] = kept
         out.write_back(fluid_in)
     return None
+
+
+def _write_group(dst, st: LiquidParticles) -> None:
+    if isinstance(dst, LiquidParticles):
+        for f in ("x_vec_", "v_vec_") + LiquidParticles.SCALARS:
+            setattr(dst, f, getattr(st, f))
+        return
+    proto = type(dst[0]) if len(dst) else None
+    if proto is None:
+        raise ValueError("cannot grow an empty particle vector of unknown type")
+    while len(dst) < len(st):
+        dst.append(proto(st.dim))
+    del dst[len(st):]
+    st.write_back(dst)
 
 
 # Python names cannot end in "!"; these aliases keep the reference's spelling
@@ -314,6 +437,8 @@ class Batch:
     """Several resident scenes stepped together (esph_batch): one launch per
     (dim, kernel, precision) class and phase for all of them."""
 
+    _close_rank = 1
+
     def __init__(self, scenes: list, graph: bool = True):
         if not scenes:
             raise ValueError("empty batch")
@@ -324,6 +449,7 @@ class Batch:
         h = c_void_p()
         self.ctx.check(self.L.esph_batch_create(self.ctx.handle, len(self.scenes), arr, byref(h)))
         self.handle = h
+        self.ctx.adopt(self)
         if graph:
             self.ctx.check(self.L.esph_batch_enable_graph(self.handle, 1))
 

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define ESPH_ABI_VERSION 1
+#define ESPH_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -65,7 +65,25 @@ enum {
     ESPH_SOLVER_FLUID_WALL = 0,                     /* FluidWallSolve.jl:8-83 */
     ESPH_SOLVER_FLUID_VELOCITY = 1,                 /* FluidVelocitySolve.jl:8-102 */
     ESPH_SOLVER_FLUID_THERMOSTATIC_WALL = 2,        /* FluidThermostaticWallSolve.jl:8-125 */
-    ESPH_SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL = 3 /* FluidCompulsiveThermostaticWallSolve.jl:8-152 */
+    ESPH_SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL = 3, /* FluidCompulsiveThermostaticWallSolve.jl:8-152 */
+    ESPH_SOLVER_FLUID_INLET_OUTLET_WALL = 4             /* FluidInletOutletWallSolve.jl:8-264 */
+};
+
+/* ---- buffer velocity profiles of the inlet/outlet solver (the reference's
+   user_defined_function, e.g. PoiseuilleFlow2D.jl:28-35,152-168) ------- */
+enum {
+    ESPH_PROFILE_NONE = 0,       /* buffer velocities are left as they are */
+    ESPH_PROFILE_CHANNEL_2D = 1, /* transient plane Poiseuille: y' = y - c[1];
+                                    u = f/(2 nu) y'(l - y') - sum_{n=0..terms} 4 f l^2 / (nu pi^3 (2n+1)^3)
+                                        sin(pi y' (2n+1) / l) exp(-(2n+1)^2 pi^2 nu t / l^2) */
+    ESPH_PROFILE_PIPE_3D = 2     /* steady pipe Poiseuille: u = f/(4 nu) (l^2 - (y-c[1])^2 - (z-c[2])^2) */
+};
+
+/* ---- particle groups (downloads) ------------------------------------------ */
+enum {
+    ESPH_GROUP_FLUID = 0,
+    ESPH_GROUP_INLET = 1, /* inlet buffer particles (inlet/outlet solver) */
+    ESPH_GROUP_OUTLET = 2 /* outlet buffer particles (inlet/outlet solver) */
 };
 
 /* ---- arithmetic forms -------------------------------------------------- */
@@ -147,6 +165,28 @@ typedef struct esph_scene_desc {
                                  check bounds, padded by one cell                       */
     double domain_min[3];
     double domain_max[3];
+    /* ESPH_SOLVER_FLUID_INLET_OUTLET_WALL: the reference's check / modify
+       closures as planes along io_axis (PoiseuilleFlow2D.jl:146-150):
+       after each step an inlet particle with x[axis] > inlet_enter is copied
+       into the fluid and moved back by inlet_shift; a fluid particle with
+       x[axis] > outlet_enter moves into the outlet; an outlet particle with
+       x[axis] > outlet_exit is removed. */
+    int32_t io_axis;
+    double inlet_enter;
+    double inlet_shift;
+    double outlet_enter;
+    double outlet_exit;
+    /* buffer velocities set after each step at t = step * dt: inlet
+       v[axis] = profile; outlet v[axis] = profile when outlet_profile is 1;
+       outlet cross components are zeroed (PoiseuilleFlow2D.jl:152-168,
+       PoiseuilleFlow3D.jl:143-158) */
+    int32_t profile;          /* ESPH_PROFILE_* */
+    int32_t profile_terms;
+    int32_t outlet_profile;
+    double profile_f;
+    double profile_nu;
+    double profile_l;         /* channel width (2D) or pipe radius (3D) */
+    double profile_center[3];
 } esph_scene_desc;
 
 typedef struct esph_fluid_arrays {
@@ -207,8 +247,12 @@ int esph_scene_create(esph_ctx* ctx, const esph_scene_desc* desc, esph_scene** o
 int esph_scene_destroy(esph_scene* scene);
 /* Wall groups are static; set them (all groups at once) before the fluid. */
 int esph_scene_set_walls(esph_scene* scene, int32_t ngroups, const esph_wall_arrays* groups);
-/* Uploads (or re-uploads) the fluid state and resets the step counter. */
+/* Uploads (or re-uploads) the fluid state and resets the step counter.  For
+   the inlet/outlet solver, set the buffers first. */
 int esph_scene_set_fluid(esph_scene* scene, const esph_fluid_arrays* fluid);
+/* Inlet and outlet buffer particles of the inlet/outlet solver (fields x, v,
+   rho, p, c, mass, gap); call before esph_scene_set_fluid. */
+int esph_scene_set_buffers(esph_scene* scene, const esph_fluid_arrays* inlet, const esph_fluid_arrays* outlet);
 /* Advances nsteps steps of eachStep! (stream-ordered, asynchronous). */
 int esph_scene_step(esph_scene* scene, int64_t nsteps);
 /* Waits for the scene's work; reports ESPH_ERR_NAN if a step met a
@@ -218,6 +262,11 @@ int esph_scene_step_count(const esph_scene* scene, int64_t* step);
 int esph_scene_fluid_count(esph_scene* scene, int64_t* n);
 /* Downloads a fluid field in reference order; capacity in doubles. */
 int esph_scene_get_fluid(esph_scene* scene, int32_t field, double* out, int64_t capacity,
+                         int64_t* written);
+/* Particle count and fields of a group (ESPH_GROUP_*), in the reference's
+   order for that group's vector. */
+int esph_scene_group_count(esph_scene* scene, int32_t group, int64_t* n);
+int esph_scene_get_group(esph_scene* scene, int32_t group, int32_t field, double* out, int64_t capacity,
                          int64_t* written);
 /* Exports the pair set the next step will use (positions as they are now),
    sorted; same output contract as esph_neighbor_list. */
@@ -268,8 +317,11 @@ enum { ESPH_REQ_QUEUED = 0, ESPH_REQ_RUNNING = 1, ESPH_REQ_DONE = 2, ESPH_REQ_FA
 typedef struct esph_server esph_server;
 int esph_server_create(esph_ctx* ctx, int32_t max_active, esph_server** out); /* max_active <= 0: no limit */
 int esph_server_destroy(esph_server* server);
+/* inlet and outlet: the buffers of ESPH_SOLVER_FLUID_INLET_OUTLET_WALL
+   requests (both NULL for the other solvers); results are the fluid group. */
 int esph_server_submit(esph_server* server, const esph_scene_desc* desc, int32_t nwall_groups,
-                       const esph_wall_arrays* walls, const esph_fluid_arrays* fluid, int64_t nsteps,
+                       const esph_wall_arrays* walls, const esph_fluid_arrays* fluid,
+                       const esph_fluid_arrays* inlet, const esph_fluid_arrays* outlet, int64_t nsteps,
                        int64_t* request_id);
 int esph_server_advance(esph_server* server, int64_t max_steps, int64_t* stepped);
 int esph_server_status(esph_server* server, int64_t request_id, int32_t* state, int64_t* steps_done);

@@ -288,6 +288,24 @@ int64_t eo_neighbor_list(int32_t dim, double cutoff, int64_t nx, const double* x
 /* Scene state (reference particle vectors as SoA, reference order)          */
 /* ------------------------------------------------------------------------ */
 
+/* A buffer particle group of the inlet/outlet solver (a Vector of
+   CommonLiquidParticle in the reference), reference order. */
+typedef struct eo_buf {
+    int64_t n, cap, next_id;
+    double *x, *v, *rho, *p, *c, *m, *gap;
+    int64_t* id;
+} eo_buf;
+
+/* Pair records fluid i -- buffer particle j (both movable: v_ij = v_i - v_j). */
+typedef struct eo_bpairs {
+    int64_t n;
+    int64_t *i, *j;
+    double *r, *W, *dW, *g, *vv; /* g, vv: dim per pair */
+} eo_bpairs;
+
+static void buf_free(eo_buf* b);
+static void bpairs_free(eo_bpairs* q);
+
 struct eo_scene {
     esph_scene_desc d;
     eo_kernel k;
@@ -307,6 +325,11 @@ struct eo_scene {
     int64_t *ffi, *ffj, *fwi, *fwj;
     double *ffr, *ffW, *ffdW, *ffg, *ffvv, *ffrv; /* g, vv, rv: dim per pair */
     double *fwr, *fwW, *fwdW, *fwg, *fwvv, *fwrv;
+    /* inlet/outlet solver (FluidInletOutletWallSolve.jl) */
+    int64_t fcap;        /* allocated fluid capacity (the fluid grows by inlet copies) */
+    int64_t next_id;     /* id of the next fluid particle pushed */
+    eo_buf inlet, outlet;
+    eo_bpairs fi, fo;    /* fluid-inlet and fluid-outlet pair records */
 };
 
 static double* dup_or_zero(const double* src, int64_t n) {
@@ -345,6 +368,10 @@ static void free_fluid(eo_scene* s) {
 void eo_scene_free(eo_scene* s) {
     if (!s) return;
     free_fluid(s);
+    buf_free(&s->inlet);
+    buf_free(&s->outlet);
+    bpairs_free(&s->fi);
+    bpairs_free(&s->fo);
     free(s->wx); free(s->wn); free(s->wgap); free(s->wv); free(s->wt); free(s->wkappa); free(s->wkind);
     free_pairs(s);
     free(s);
@@ -388,8 +415,78 @@ int eo_scene_set_fluid(eo_scene* s, const esph_fluid_arrays* f) {
     s->cp = dup_or_zero(f->cp, n); s->sw = dup_or_zero(NULL, n); s->ss = dup_or_zero(NULL, n);
     s->id = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
     for (int64_t i = 0; i < n; ++i) s->id[i] = i;
+    s->fcap = n > 0 ? n : 1;
+    s->next_id = n;
     s->step = 0;
     return 0;
+}
+
+static void* grow(void* p, int64_t n, size_t elem) {
+    void* q = realloc(p, (size_t)(n > 0 ? n : 1) * elem);
+    return q;
+}
+
+/* Makes room for `need` fluid particles (push! onto the fluid vector). */
+static void fluid_reserve(eo_scene* s, int64_t need) {
+    if (need <= s->fcap) return;
+    int64_t cap = s->fcap;
+    while (cap < need) cap *= 2;
+    const int D = s->d.dim;
+    double** vec[] = {&s->x, &s->v, &s->dv};
+    for (int q = 0; q < 3; ++q) *vec[q] = (double*)grow(*vec[q], cap * D, sizeof(double));
+    double** sc[] = {&s->rho, &s->drho, &s->p, &s->c, &s->m, &s->gap, &s->t, &s->dtt, &s->kappa, &s->cp, &s->sw, &s->ss};
+    for (int q = 0; q < 12; ++q) *sc[q] = (double*)grow(*sc[q], cap, sizeof(double));
+    s->id = (int64_t*)grow(s->id, cap, sizeof(int64_t));
+    s->fcap = cap;
+}
+
+static void buf_free(eo_buf* b) {
+    free(b->x); free(b->v); free(b->rho); free(b->p); free(b->c); free(b->m); free(b->gap); free(b->id);
+    memset(b, 0, sizeof(*b));
+}
+
+static void buf_reserve(eo_buf* b, int D, int64_t need) {
+    if (need <= b->cap) return;
+    int64_t cap = b->cap > 0 ? b->cap : 1;
+    while (cap < need) cap *= 2;
+    b->x = (double*)grow(b->x, cap * D, sizeof(double));
+    b->v = (double*)grow(b->v, cap * D, sizeof(double));
+    b->rho = (double*)grow(b->rho, cap, sizeof(double));
+    b->p = (double*)grow(b->p, cap, sizeof(double));
+    b->c = (double*)grow(b->c, cap, sizeof(double));
+    b->m = (double*)grow(b->m, cap, sizeof(double));
+    b->gap = (double*)grow(b->gap, cap, sizeof(double));
+    b->id = (int64_t*)grow(b->id, cap, sizeof(int64_t));
+    b->cap = cap;
+}
+
+static void buf_set(eo_buf* b, int D, const esph_fluid_arrays* f) {
+    buf_free(b);
+    const int64_t n = f ? f->n : 0;
+    buf_reserve(b, D, n > 0 ? n : 1);
+    b->n = n;
+    if (n > 0) {
+        memcpy(b->x, f->x, sizeof(double) * (size_t)(n * D));
+        memcpy(b->v, f->v, sizeof(double) * (size_t)(n * D));
+        memcpy(b->rho, f->rho, sizeof(double) * (size_t)n);
+        memcpy(b->p, f->p, sizeof(double) * (size_t)n);
+        memcpy(b->c, f->c, sizeof(double) * (size_t)n);
+        memcpy(b->m, f->mass, sizeof(double) * (size_t)n);
+        memcpy(b->gap, f->gap, sizeof(double) * (size_t)n);
+    }
+    for (int64_t i = 0; i < n; ++i) b->id[i] = i;
+    b->next_id = n;
+}
+
+int eo_scene_set_buffers(eo_scene* s, const esph_fluid_arrays* inlet, const esph_fluid_arrays* outlet) {
+    buf_set(&s->inlet, s->d.dim, inlet);
+    buf_set(&s->outlet, s->d.dim, outlet);
+    return 0;
+}
+
+static void bpairs_free(eo_bpairs* q) {
+    free(q->i); free(q->j); free(q->r); free(q->W); free(q->dW); free(q->g); free(q->vv);
+    memset(q, 0, sizeof(*q));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -486,6 +583,209 @@ static void update_density_pressure(eo_scene* s, int64_t i) {
     s->c[i] = s->d.c_0 * sqrt(ipow(rr, s->d.gamma - 1));
 }
 
+/* Pair records fluid i -- buffer particle j for the inlet/outlet solver
+   (findNeighbours!(…, fluid, inlet/outlet, …), FluidInletOutletWallSolve.jl:41-61). */
+static void make_buffer_records(eo_scene* s, const eo_buf* b, eo_bpairs* out, int nthreads) {
+    const int D = s->d.dim;
+    bpairs_free(out);
+    int64_t *start, *nbr;
+    neighbour_csr(D, s->k.influence_radius, s->n, s->x, b->n, b->x, 0, 0, nthreads, &start, &nbr);
+    const int64_t m = start[s->n];
+    out->n = m;
+    out->i = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+    out->j = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+    out->r = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    out->W = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    out->dW = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    out->g = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m * D : 1));
+    out->vv = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m * D : 1));
+    for (int64_t i = 0; i < s->n; ++i)
+        for (int64_t q = start[i]; q < start[i + 1]; ++q) {
+            const int64_t j = nbr[q];
+            out->i[q] = i;
+            out->j[q] = j;
+            const double d2 = dist2(D, s->x + i * D, b->x + j * D);
+            const double rr = sqrt(d2);
+            out->r[q] = rr;
+            out->W[q] = eo_kernel_value(&s->k, rr);
+            out->dW[q] = eo_kernel_gradient(&s->k, rr);
+            double rv[3], nrm2 = 0.0;
+            for (int k = 0; k < D; ++k) {
+                rv[k] = s->x[i * D + k] - b->x[j * D + k];
+                nrm2 = nrm2 + rv[k] * rv[k];
+                out->vv[q * D + k] = s->v[i * D + k] - b->v[j * D + k];
+            }
+            const double nrm = sqrt(nrm2), inv = 1.0 / nrm;
+            for (int k = 0; k < D; ++k) {
+                double gk = out->dW[q] * (rv[k] * inv);
+                if (nrm == 0.0 || isnan(gk)) gk = 0.0;
+                out->g[q * D + k] = gk;
+            }
+        }
+    free(start);
+    free(nbr);
+}
+
+/* continuity!(fluid_i, buffer_j, …) (Continuity.jl:8-17); the buffer's own
+   drho is reset afterwards (FluidInletOutletWallSolve.jl:91-99), so only
+   the fluid side is kept. */
+static void buffer_continuity(eo_scene* s, const eo_buf* b, const eo_bpairs* q) {
+    const int D = s->d.dim;
+    for (int64_t e = 0; e < q->n; ++e) {
+        const int64_t i = q->i[e], j = q->j[e];
+        double drho = 0.0;
+        for (int k = 0; k < D; ++k) drho = drho + q->vv[e * D + k] * q->g[e * D + k];
+        s->drho[i] += drho * b->m[j];
+    }
+}
+
+/* pressureForce!/viscosityForce!(fluid_i, buffer_j, …), the fluid-fluid forms
+   (PressureForce.jl:8-21, ViscosityForce.jl:8-25); the buffer side is reset. */
+static void buffer_forces(eo_scene* s, const eo_buf* b, const eo_bpairs* q) {
+    const int D = s->d.dim;
+    const double h = s->k.h;
+    for (int64_t e = 0; e < q->n; ++e) {
+        const int64_t i = q->i[e], j = q->j[e];
+        const double* g = q->g + e * D;
+        const double* vv = q->vv + e * D;
+        const double r = q->r[e], W = q->W[e], dW = q->dW[e];
+        double p_rho_2 = s->p[i] / (s->rho[i] * s->rho[i]) + b->p[j] / (b->rho[j] * b->rho[j]);
+        p_rho_2 += fabs(p_rho_2) * 0.01 * W / eo_kernel_value(&s->k, (s->gap[i] + b->gap[j]) / 2.0);
+        for (int k = 0; k < D; ++k) s->dv[i * D + k] += -b->m[j] * p_rho_2 * g[k];
+        const double sum_rho = s->rho[i] + b->rho[j];
+        const double mu_i = s->nu_0 * s->rho[i];
+        const double mu_j = s->nu_0 * b->rho[j];
+        const double vf = 4.0 * (mu_i + mu_j) * r * dW / (sum_rho * sum_rho) / (r * r + 0.01 * (h * h));
+        for (int k = 0; k < D; ++k) s->dv[i * D + k] += b->m[j] * vf * vv[k];
+    }
+}
+
+/* The buffer velocity profile (user_defined_function of the examples):
+   PoiseuilleFlow2D.jl:28-35 (uxTheory(y, t)), PoiseuilleFlow3D.jl:32-34. */
+static double profile_u(const eo_scene* s, const double* x, double t) {
+    const esph_scene_desc* d = &s->d;
+    const double f = d->profile_f, nu = d->profile_nu, l = d->profile_l;
+    if (d->profile == ESPH_PROFILE_CHANNEL_2D) {
+        const double PI = 3.14159265358979323846;
+        const double y = x[1] - d->profile_center[1];
+        double ux = f / 2.0 / nu * y * (l - y);
+        for (int n = 0; n <= d->profile_terms; ++n) {
+            const double o = (double)(2 * n + 1);
+            ux -= 4.0 * f * (l * l) / nu / (PI * PI * PI) / (o * o * o) * sin(PI * y / l * o) *
+                  exp(-(o * o) * (PI * PI) * nu * t / (l * l));
+        }
+        return ux;
+    }
+    if (d->profile == ESPH_PROFILE_PIPE_3D) {
+        const double y = x[1] - d->profile_center[1], z = x[2] - d->profile_center[2];
+        return f / 4.0 / nu * (l * l - y * y - z * z);
+    }
+    return 0.0;
+}
+
+static void buf_push(eo_buf* dst, int D, const double* x, const double* v, double rho, double p, double c,
+                     double m, double gap) {
+    buf_reserve(dst, D, dst->n + 1);
+    const int64_t w = dst->n++;
+    for (int k = 0; k < D; ++k) {
+        dst->x[w * D + k] = x[k];
+        dst->v[w * D + k] = v[k];
+    }
+    dst->rho[w] = rho; dst->p[w] = p; dst->c[w] = c; dst->m[w] = m; dst->gap[w] = gap;
+    dst->id[w] = dst->next_id++;
+}
+
+/* Inlet/outlet migration and the buffer velocity update after a step
+   (FluidInletOutletWallSolve.jl:196-262): collect the three index sets, push
+   inlet copies onto the fluid and fluid copies onto the outlet, delete (in
+   order) the fluid particles that left and the outlet particles that exited,
+   move the entered inlet particles back, then set buffer velocities at
+   t = step * dt. */
+static void migrate(eo_scene* s) {
+    const int D = s->d.dim, a = s->d.io_axis;
+    eo_buf* in = &s->inlet;
+    eo_buf* out = &s->outlet;
+    int64_t n_in = 0, n_fo = 0, n_ex = 0;
+    int64_t* ids_in = (int64_t*)malloc(sizeof(int64_t) * (size_t)(in->n > 0 ? in->n : 1));
+    int64_t* ids_fo = (int64_t*)malloc(sizeof(int64_t) * (size_t)(s->n > 0 ? s->n : 1));
+    int64_t* ids_ex = (int64_t*)malloc(sizeof(int64_t) * (size_t)(out->n > 0 ? out->n : 1));
+    for (int64_t i = 0; i < in->n; ++i)
+        if (in->x[i * D + a] > s->d.inlet_enter) ids_in[n_in++] = i;
+    for (int64_t i = 0; i < s->n; ++i)
+        if (s->x[i * D + a] > s->d.outlet_enter) ids_fo[n_fo++] = i;
+    for (int64_t i = 0; i < out->n; ++i)
+        if (out->x[i * D + a] > s->d.outlet_exit) ids_ex[n_ex++] = i;
+    /* push inlet copies onto the fluid (deepcopy: dv and drho are zero) */
+    fluid_reserve(s, s->n + n_in);
+    for (int64_t e = 0; e < n_in; ++e) {
+        const int64_t i = ids_in[e], w = s->n++;
+        for (int k = 0; k < D; ++k) {
+            s->x[w * D + k] = in->x[i * D + k];
+            s->v[w * D + k] = in->v[i * D + k];
+            s->dv[w * D + k] = 0.0;
+        }
+        s->rho[w] = in->rho[i]; s->drho[w] = 0.0; s->p[w] = in->p[i]; s->c[w] = in->c[i];
+        s->m[w] = in->m[i]; s->gap[w] = in->gap[i];
+        s->t[w] = 0.0; s->dtt[w] = 0.0; s->kappa[w] = 0.0; s->cp[w] = 0.0; s->sw[w] = 0.0; s->ss[w] = 0.0;
+        s->id[w] = s->next_id++;
+    }
+    /* push fluid copies onto the outlet */
+    for (int64_t e = 0; e < n_fo; ++e) {
+        const int64_t i = ids_fo[e];
+        buf_push(out, D, s->x + i * D, s->v + i * D, s->rho[i], s->p[i], s->c[i], s->m[i], s->gap[i]);
+    }
+    /* deleteat!(fluid, ids_fo), stable */
+    if (n_fo > 0) {
+        int64_t w = 0, e = 0;
+        for (int64_t i = 0; i < s->n; ++i) {
+            if (e < n_fo && ids_fo[e] == i) { ++e; continue; }
+            if (w != i) {
+                for (int k = 0; k < D; ++k) {
+                    s->x[w * D + k] = s->x[i * D + k];
+                    s->v[w * D + k] = s->v[i * D + k];
+                    s->dv[w * D + k] = s->dv[i * D + k];
+                }
+                s->rho[w] = s->rho[i]; s->drho[w] = s->drho[i]; s->p[w] = s->p[i]; s->c[w] = s->c[i];
+                s->m[w] = s->m[i]; s->gap[w] = s->gap[i]; s->id[w] = s->id[i];
+            }
+            ++w;
+        }
+        s->n = w;
+    }
+    /* deleteat!(outlet, ids_ex), stable */
+    if (n_ex > 0) {
+        int64_t w = 0, e = 0;
+        for (int64_t i = 0; i < out->n; ++i) {
+            if (e < n_ex && ids_ex[e] == i) { ++e; continue; }
+            if (w != i) {
+                for (int k = 0; k < D; ++k) {
+                    out->x[w * D + k] = out->x[i * D + k];
+                    out->v[w * D + k] = out->v[i * D + k];
+                }
+                out->rho[w] = out->rho[i]; out->p[w] = out->p[i]; out->c[w] = out->c[i];
+                out->m[w] = out->m[i]; out->gap[w] = out->gap[i]; out->id[w] = out->id[i];
+            }
+            ++w;
+        }
+        out->n = w;
+    }
+    /* inlet_into_calculation_domain_modify: move back along the axis */
+    for (int64_t e = 0; e < n_in; ++e) in->x[ids_in[e] * D + a] -= s->d.inlet_shift;
+    free(ids_in);
+    free(ids_fo);
+    free(ids_ex);
+    /* user_defined_function(t = step * dt) */
+    if (s->d.profile != ESPH_PROFILE_NONE) {
+        const double t = (double)s->step * s->d.dt;
+        for (int64_t i = 0; i < in->n; ++i) in->v[i * D + a] = profile_u(s, in->x + i * D, t);
+        for (int64_t i = 0; i < out->n; ++i) {
+            if (s->d.outlet_profile) out->v[i * D + a] = profile_u(s, out->x + i * D, t);
+            for (int k = 0; k < D; ++k)
+                if (k != a) out->v[i * D + k] = 0.0;
+        }
+    }
+}
+
 static void step_once(eo_scene* s, int nthreads) {
     const int D = s->d.dim;
     const double h = s->k.h;
@@ -495,6 +795,11 @@ static void step_once(eo_scene* s, int nthreads) {
     /* neighbours (eachStep! :21-35 of FluidWallSolve.jl and the equivalents) */
     make_records(s, 0, nthreads);
     if (s->nw > 0) make_records(s, 1, nthreads);
+    const int io = solver == ESPH_SOLVER_FLUID_INLET_OUTLET_WALL;
+    if (io) {
+        make_buffer_records(s, &s->inlet, &s->fi, nthreads);
+        make_buffer_records(s, &s->outlet, &s->fo, nthreads);
+    }
 
     /* continuity! over FF pairs, Continuity.jl:8-17 */
 #pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
@@ -504,6 +809,10 @@ static void step_once(eo_scene* s, int nthreads) {
         for (int k = 0; k < D; ++k) drho = drho + s->ffvv[q * D + k] * s->ffg[q * D + k];
         atomic_add(&s->drho[i], drho * s->m[j]);
         atomic_add(&s->drho[j], drho * s->m[i]);
+    }
+    if (io) {
+        buffer_continuity(s, &s->inlet, &s->fi);
+        buffer_continuity(s, &s->outlet, &s->fo);
     }
     /* updateDensity! + updatePressure! */
 #pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
@@ -547,7 +856,8 @@ static void step_once(eo_scene* s, int nthreads) {
        FluidVelocitySolve.jl:60-82 and FluidThermostaticWallSolve.jl:74-103,
        FluidCompulsiveThermostaticWallSolve.jl:87-129 add pressureForce!(wall);
        thermostatic walls of the thermal solvers add thermalConduction!. */
-    const int wall_pressure = solver != ESPH_SOLVER_FLUID_WALL;
+    /* the inlet/outlet solver has no wall pressure term (FluidInletOutletWallSolve.jl:121-122) */
+    const int wall_pressure = solver != ESPH_SOLVER_FLUID_WALL && !io;
 #pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
     for (int64_t q = 0; q < s->nfw; ++q) {
         int64_t i = s->fwi[q], j = s->fwj[q];
@@ -596,6 +906,11 @@ static void step_once(eo_scene* s, int nthreads) {
         }
     }
 
+    if (io) {
+        buffer_forces(s, &s->inlet, &s->fi);
+        buffer_forces(s, &s->outlet, &s->fo);
+    }
+
     /* integration: updateTemperature!, updateVelocity!(…, body_force),
        updatePosition! (UpdateTemperature.jl:8-12, UpdateVelocity.jl:8-23,
        UpdatePosition.jl:8-14; FluidWallSolve.jl:82-85,
@@ -618,6 +933,14 @@ static void step_once(eo_scene* s, int nthreads) {
         }
     }
 
+    if (io) {
+        /* buffers: dv = 0, then updatePosition! (FluidInletOutletWallSolve.jl:170-186) */
+        eo_buf* bs[2] = {&s->inlet, &s->outlet};
+        for (int q = 0; q < 2; ++q)
+            for (int64_t i = 0; i < bs[q]->n; ++i)
+                for (int k = 0; k < D; ++k)
+                    bs[q]->x[i * D + k] += bs[q]->v[i * D + k] * dt - 0.0 * (dt * dt) / 2.0;
+    }
     s->step += 1;
     /* reconstructScalar!(fluid, :rho_, FF pairs, kernel), ReconstructValue.jl:22-57,
        gated by isDensityReinitializedStep (FixedTimeIntervalDiscretization.jl:18-23) */
@@ -640,8 +963,9 @@ static void step_once(eo_scene* s, int nthreads) {
             s->ss[i] = 0.0;
         }
     }
+    if (io) migrate(s);
     /* check + deleteat! (FluidWallSolve.jl:89-97): stable removal */
-    if (s->d.check_enabled && s->d.check_interval > 0 && s->step % s->d.check_interval == 0) {
+    if (!io && s->d.check_enabled && s->d.check_interval > 0 && s->step % s->d.check_interval == 0) {
         int64_t w = 0;
         for (int64_t i = 0; i < s->n; ++i) {
             int out = 0;
@@ -696,6 +1020,35 @@ int eo_scene_get_fluid(const eo_scene* s, int32_t field, double* out) {
         default: return -1;
     }
     memcpy(out, src, sizeof(double) * (size_t)w);
+    return 0;
+}
+
+int64_t eo_scene_group_count(const eo_scene* s, int32_t group) {
+    if (group == ESPH_GROUP_INLET) return s->inlet.n;
+    if (group == ESPH_GROUP_OUTLET) return s->outlet.n;
+    return s->n;
+}
+
+int eo_scene_get_group(const eo_scene* s, int32_t group, int32_t field, double* out) {
+    if (group == ESPH_GROUP_FLUID) return eo_scene_get_fluid(s, field, out);
+    const eo_buf* b = group == ESPH_GROUP_INLET ? &s->inlet : &s->outlet;
+    const int D = s->d.dim;
+    const double* src = NULL;
+    int64_t w = b->n;
+    switch (field) {
+        case ESPH_FIELD_X: src = b->x; w = b->n * D; break;
+        case ESPH_FIELD_V: src = b->v; w = b->n * D; break;
+        case ESPH_FIELD_RHO: src = b->rho; break;
+        case ESPH_FIELD_P: src = b->p; break;
+        case ESPH_FIELD_C: src = b->c; break;
+        case ESPH_FIELD_MASS: src = b->m; break;
+        case ESPH_FIELD_GAP: src = b->gap; break;
+        case ESPH_FIELD_ID:
+            for (int64_t i = 0; i < b->n; ++i) out[i] = (double)b->id[i];
+            return 0;
+        default: return -1;
+    }
+    if (w > 0) memcpy(out, src, sizeof(double) * (size_t)w);
     return 0;
 }
 

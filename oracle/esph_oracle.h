@@ -50,6 +50,10 @@ eo_scene* eo_scene_new(const esph_scene_desc* desc);
 void eo_scene_free(eo_scene* s);
 int eo_scene_set_walls(eo_scene* s, int32_t ngroups, const esph_wall_arrays* groups);
 int eo_scene_set_fluid(eo_scene* s, const esph_fluid_arrays* f);
+/* Inlet/outlet buffers of ESPH_SOLVER_FLUID_INLET_OUTLET_WALL. */
+int eo_scene_set_buffers(eo_scene* s, const esph_fluid_arrays* inlet, const esph_fluid_arrays* outlet);
+int64_t eo_scene_group_count(const eo_scene* s, int32_t group);
+int eo_scene_get_group(const eo_scene* s, int32_t group, int32_t field, double* out);
 /* Runs nsteps of eachStep!; returns 0. */
 int eo_scene_step(eo_scene* s, int64_t nsteps, int32_t nthreads);
 int64_t eo_scene_fluid_count(const eo_scene* s);

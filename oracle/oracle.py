@@ -41,6 +41,9 @@ def lib():
             "eo_scene_free": ([vp], None),
             "eo_scene_set_walls": ([vp, c_int32, POINTER(WallArrays)], c_int32),
             "eo_scene_set_fluid": ([vp, POINTER(FluidArrays)], c_int32),
+            "eo_scene_set_buffers": ([vp, POINTER(FluidArrays), POINTER(FluidArrays)], c_int32),
+            "eo_scene_group_count": ([vp, c_int32], c_int64),
+            "eo_scene_get_group": ([vp, c_int32, c_int32, DP], c_int32),
             "eo_scene_step": ([vp, c_int64, c_int32], c_int32),
             "eo_scene_fluid_count": ([vp], c_int64),
             "eo_scene_step_count": ([vp], c_int64),
@@ -87,8 +90,8 @@ class OracleScene:
     """FP64 CPU restatement of one scene (same descriptor and arrays as the
     device Scene)."""
 
-    def __init__(self, desc, fluid, walls):
-        from ethersph._lib import FluidArrays, WallArrays
+    def __init__(self, desc, fluid, walls, buffers=None):
+        from ethersph._lib import WallArrays
 
         self.L = lib()
         self.dim = desc.dim
@@ -104,19 +107,13 @@ class OracleScene:
             arr[q].kind = w.kind
             arr[q].n = len(w)
             arr[q].x, arr[q].normal, arr[q].gap, arr[q].v, arr[q].t, arr[q].kappa = (_p(v) for v in a)
-        self.L.eo_scene_set_walls(self.h, len(walls), arr)
-        f = FluidArrays()
-        keep = {k: np.ascontiguousarray(getattr(fluid, k), dtype=np.float64)
-                for k in ("x_vec_", "v_vec_", "rho_", "p_", "c_", "mass_", "gap_")}
-        if self.thermal:
-            for k in ("t_", "kappa_", "cp_"):
-                keep[k] = np.ascontiguousarray(getattr(fluid, k), dtype=np.float64)
-        f.n = len(fluid)
-        f.x, f.v, f.rho, f.p, f.c = (_p(keep[k]) for k in ("x_vec_", "v_vec_", "rho_", "p_", "c_"))
-        f.mass, f.gap = _p(keep["mass_"]), _p(keep["gap_"])
-        if self.thermal:
-            f.t, f.kappa, f.cp = _p(keep["t_"]), _p(keep["kappa_"]), _p(keep["cp_"])
-        self.L.eo_scene_set_fluid(self.h, byref(f))
+        assert self.L.eo_scene_set_walls(self.h, len(walls), arr) == 0
+        if buffers is not None:
+            a, k1 = _fluid_arrays(buffers[0], False)
+            b, k2 = _fluid_arrays(buffers[1], False)
+            assert self.L.eo_scene_set_buffers(self.h, byref(a), byref(b)) == 0
+        f, keep = _fluid_arrays(fluid, self.thermal)
+        assert self.L.eo_scene_set_fluid(self.h, byref(f)) == 0
 
     def step(self, n: int, nthreads: int = 1) -> None:
         self.L.eo_scene_step(self.h, n, nthreads)
@@ -124,11 +121,17 @@ class OracleScene:
     def fluid_count(self) -> int:
         return int(self.L.eo_scene_fluid_count(self.h))
 
-    def get(self, field: int) -> np.ndarray:
-        n = self.fluid_count()
+    def group_count(self, group: int) -> int:
+        return int(self.L.eo_scene_group_count(self.h, group))
+
+    def get(self, field: int, group: int = 0) -> np.ndarray:
+        n = self.group_count(group)
         w = self.dim if field in (0, 1) else 1
         out = np.empty(max(n * w, 1))
-        self.L.eo_scene_get_fluid(self.h, field, _p(out))
+        if group == 0:
+            assert self.L.eo_scene_get_fluid(self.h, field, _p(out)) == 0
+        else:
+            assert self.L.eo_scene_get_group(self.h, group, field, _p(out)) == 0
         out = out[: n * w]
         return out.reshape(n, w) if w > 1 else out
 
@@ -150,5 +153,22 @@ class OracleScene:
             pass
 
 
+def _fluid_arrays(fluid, thermal):
+    from ethersph._lib import FluidArrays
+
+    f = FluidArrays()
+    keep = {k: np.ascontiguousarray(getattr(fluid, k), dtype=np.float64)
+            for k in ("x_vec_", "v_vec_", "rho_", "p_", "c_", "mass_", "gap_")}
+    if thermal:
+        for k in ("t_", "kappa_", "cp_"):
+            keep[k] = np.ascontiguousarray(getattr(fluid, k), dtype=np.float64)
+    f.n = len(fluid)
+    f.x, f.v, f.rho, f.p, f.c = (_p(keep[k]) for k in ("x_vec_", "v_vec_", "rho_", "p_", "c_"))
+    f.mass, f.gap = _p(keep["mass_"]), _p(keep["gap_"])
+    if thermal:
+        f.t, f.kappa, f.cp = _p(keep["t_"]), _p(keep["kappa_"]), _p(keep["cp_"])
+    return f, keep
+
+
 def from_scenario(sc, precision: int = 0) -> OracleScene:
-    return OracleScene(sc.desc(precision), sc.fluid, sc.walls)
+    return OracleScene(sc.desc(precision), sc.fluid, sc.walls, getattr(sc, "buffers", None))

@@ -88,3 +88,78 @@ def rel_l2(a, b):
 
 
 __all__ = ["lattice_scene", "interior", "random_blob", "wall_plane", "rel_l2", "AABBCheck"]
+
+
+def free_io_scene(u=1.0, dt=1e-3, R=0.03, profile=None):
+    """Inlet/outlet scene of isolated particles (pairwise farther apart than
+    R at every step, no body force): every particle moves at its constant
+    velocity, so the migrations follow from the positions alone.
+
+    inlet_enter = 0, inlet_shift = 0.1, outlet_enter = 1.0, outlet_exit = 1.02
+    along x; u * dt = 1e-3 per step."""
+    from ethersph.solve import InletOutletPlanes
+
+    dim = 2
+    sk = SmoothKernel(R, dim, KERNELS["CubicSpline"])
+    model = CommonWeaklyCompressibleLiquidModel(1000.0, 10.0, 0.0, 1, 1e-3, [0.0, 0.0])
+    td = DensityReinitializedFixedTimeIntervalForwardEuler(dt, 1, 1, 0, 1000)
+
+    def group(X, V):
+        p = LiquidParticles(dim, len(X))
+        p.x_vec_ = np.asarray(X, dtype=np.float64)
+        p.v_vec_ = np.asarray(V, dtype=np.float64)
+        p.rho_ = np.full(len(p), 1000.0)
+        p.c_ = np.full(len(p), 10.0)
+        p.mass_ = np.full(len(p), 0.1)
+        p.gap_ = np.full(len(p), 0.01)
+        return p
+
+    # fluid: two leave together at step 50 (x0 + 50 u dt > 1.0), one at 60
+    fluid = group([[0.9505, 0.50], [0.9405, 0.60], [0.9505, 0.40], [0.3, 0.9]],
+                  [[u, 0.0], [u, 0.0], [u, 0.0], [0.0, 0.0]])
+    # inlet: two enter together at step 21, one at 31; each re-enters after
+    # 100 more steps (shift 0.1)
+    inlet = group([[-0.0205, 0.20], [-0.0305, 0.10], [-0.0205, 0.05]], [[u, 0.0], [u, 0.0], [u, 0.0]])
+    # outlet: one leaves at step 11 (x > 1.02), one stays put
+    outlet = group([[1.0105, 0.70], [1.01, 0.85]], [[u, 0.0], [0.0, 0.0]])
+    w = WallParticles(_lib.WALL_COMPULSIVE, dim, 1)
+    w.x_vec_ = np.array([[5.0, 5.0]])
+    w.normal_vec_ = np.array([[0.0, 1.0]])
+    w.gap_ = np.array([0.01])
+    planes = InletOutletPlanes(0, 0.0, 0.1, 1.0, 1.02)
+    return Scenario("free_io", dim, sk, model, td, fluid, [w], None, (inlet, outlet), planes, profile)
+
+
+def migrate_by_lists(fluid, inlet, outlet, steps, dt, planes):
+    """The migration rules of FluidInletOutletWallSolve.jl:193-262 on plain
+    Python lists of free particles (x, v, id): per step every particle moves
+    x += v dt; then the three predicates are evaluated on the moved
+    positions; inlet copies are push!ed onto the fluid in inlet order, the
+    leaving fluid particles push!ed onto the outlet in fluid order, the
+    leavers and the expired outlet particles removed with deleteat!, and the
+    entering inlet particles moved back."""
+    fl = [[np.array(x, dtype=np.float64), np.array(v, dtype=np.float64), i]
+          for i, (x, v) in enumerate(zip(fluid.x_vec_, fluid.v_vec_))]
+    il = [[np.array(x, dtype=np.float64), np.array(v, dtype=np.float64), i]
+          for i, (x, v) in enumerate(zip(inlet.x_vec_, inlet.v_vec_))]
+    ol = [[np.array(x, dtype=np.float64), np.array(v, dtype=np.float64), i]
+          for i, (x, v) in enumerate(zip(outlet.x_vec_, outlet.v_vec_))]
+    next_f, next_o = len(fl), len(ol)
+    a = planes.axis
+    for _ in range(steps):
+        for p in fl + il + ol:
+            p[0] = p[0] + p[1] * dt - 0.0 * (dt * dt) / 2.0
+        enter = [k for k, p in enumerate(il) if p[0][a] > planes.inlet_enter]
+        leave = [k for k, p in enumerate(fl) if p[0][a] > planes.outlet_enter]
+        gone = [k for k, p in enumerate(ol) if p[0][a] > planes.outlet_exit]
+        for k in enter:
+            fl.append([il[k][0].copy(), il[k][1].copy(), next_f])
+            next_f += 1
+        for k in leave:
+            ol.append([fl[k][0].copy(), fl[k][1].copy(), next_o])
+            next_o += 1
+        fl = [p for k, p in enumerate(fl) if k not in set(leave)]
+        ol = [p for k, p in enumerate(ol) if k not in set(gone)]
+        for k in enter:
+            il[k][0][a] -= planes.inlet_shift
+    return fl, il, ol

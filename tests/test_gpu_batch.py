@@ -69,12 +69,14 @@ def test_batch_without_graph_and_members_stepped_alone(ctx):
     b.step(30)
     mixed[0].step(10)
     mixed[2].step(10)
-    alone[1].step(30)
+    alone[1].step(10)
     alone[0].step(10)
     alone[2].step(10)
     b.sync()
     for a, m in zip(alone, mixed):
+        assert m.step_count == a.step_count == 60
         assert_same(a, m)
+    b.close()
 
 
 def test_scene_graph_equals_plain(ctx):
