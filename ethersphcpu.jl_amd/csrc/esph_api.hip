@@ -147,8 +147,16 @@ extern "C" int esph_scene_create(esph_ctx* ctx, const esph_scene_desc* desc, esp
     if (!(d.influence_radius > 0) || !kernel_params(d.kernel, d.dim, d.influence_radius, &h, &sigma))
         return fail(ctx, ESPH_ERR_ARG, "bad kernel or influence_radius");
     if (d.kernel == ESPH_WENDLAND_C2 && d.dim == 1) return fail(ctx, ESPH_ERR_ARG, "WendlandC2 has no 1D form");
-    if (d.solver < ESPH_SOLVER_FLUID_WALL || d.solver > ESPH_SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL)
+    if (d.solver < ESPH_SOLVER_FLUID_WALL || d.solver > ESPH_SOLVER_FLUID_INLET_OUTLET_WALL)
         return fail(ctx, ESPH_ERR_ARG, "bad solver");
+    if (d.solver == ESPH_SOLVER_FLUID_INLET_OUTLET_WALL) {
+        if (d.io_axis < 0 || d.io_axis >= d.dim) return fail(ctx, ESPH_ERR_ARG, "io_axis must be 0..dim-1");
+        if (d.profile < ESPH_PROFILE_NONE || d.profile > ESPH_PROFILE_PIPE_3D ||
+            (d.profile == ESPH_PROFILE_CHANNEL_2D && d.dim != 2) || (d.profile == ESPH_PROFILE_PIPE_3D && d.dim != 3))
+            return fail(ctx, ESPH_ERR_ARG, "bad buffer profile for this dim");
+        if (d.profile != ESPH_PROFILE_NONE && (!(d.profile_nu > 0) || !(d.profile_l > 0) || d.profile_terms < 0))
+            return fail(ctx, ESPH_ERR_ARG, "profile needs nu > 0, l > 0 and terms >= 0");
+    }
     if (d.precision != ESPH_FP64 && d.precision != ESPH_FP32_BULK) return fail(ctx, ESPH_ERR_ARG, "bad precision");
     if (d.gamma < 1) return fail(ctx, ESPH_ERR_ARG, "gamma must be >= 1");
     if (!(d.rho_0 > 0) || !(d.dt > 0)) return fail(ctx, ESPH_ERR_ARG, "rho_0 and dt must be positive");
@@ -892,6 +900,38 @@ extern "C" int esph_scene_neighbours(esph_scene* s, int32_t pairset, int32_t* ou
         out_pairs[2 * q] = (int32_t)(pairs[q] >> 32);
         out_pairs[2 * q + 1] = (int32_t)(pairs[q] & 0xffffffff);
     }
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_probe(esph_scene* s, int32_t field, int64_t npoints, const double* points,
+                                double none_value, double* out) {
+    if (!s || npoints < 0 || (npoints > 0 && (!points || !out)))
+        return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "probe: bad argument");
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    const bool thermal = s->h.flags & FLAG_THERMAL;
+    if (field != ESPH_FIELD_V && field != ESPH_FIELD_RHO && field != ESPH_FIELD_P && field != ESPH_FIELD_C &&
+        !(field == ESPH_FIELD_T && thermal))
+        return fail(s->ctx, ESPH_ERR_ARG, "probe: field must be V, RHO, P, C or (thermal solvers) T");
+    if (npoints == 0) return ESPH_OK;
+    esph_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int D = s->d.dim;
+    const int w = field == ESPH_FIELD_V ? D : 1;
+    hipStream_t st = ctx->stream;
+    Arena a;
+    double* dp = (double*)a.get(sizeof(double) * (size_t)npoints * D);
+    double* dout = (double*)a.get(sizeof(double) * (size_t)npoints * w);
+    if (!dp || !dout) {
+        a.release();
+        return fail(ctx, ESPH_ERR_HIP, "probe: allocation failed");
+    }
+    hipError_t e = hipMemcpyAsync(dp, points, sizeof(double) * (size_t)npoints * D, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_probe(scene_bucket(s), npoints, dp, field, none_value, dout, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, sizeof(double) * (size_t)npoints * w, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    a.release();
+    if (e != hipSuccess) return hip_fail(ctx, "probe", e);
     return ESPH_OK;
 }
 

@@ -934,6 +934,41 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const SceneDev* __restrict__ s
     }
 }
 
+// kernelValueInterpolation (DataIO/VTPToDataFrame.jl:77-142) over the
+// resident fluid: one probe point per thread, the probe's 3^D cells of the
+// current cell list, FP64 sums of W m_j / rho_j and W m_j / rho_j f_j in cell
+// order; none_value where no fluid particle is within R.
+template <int D, int K>
+__global__ __launch_bounds__(kBlock) void k_probe(const SceneDev* __restrict__ sd, int64_t np,
+                                                  const double* __restrict__ pts, int field, double none_value,
+                                                  double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const SceneDev& s = *sd;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const SceneDyn dyn = *s.dyn;
+    const FluidSet& S = s.set[dyn.parity];
+    const double* f = S.rho;
+    int w = 1;
+    if (field == ESPH_FIELD_V) { f = S.v; w = D; }
+    else if (field == ESPH_FIELD_P) f = S.p;
+    else if (field == ESPH_FIELD_C) f = S.c;
+    else if (field == ESPH_FIELD_T) f = S.T;
+    double xi[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < D; ++k) xi[k] = pts[q * D + k];
+    int c[3];
+    cell_of<D>(s, xi, c);
+    double sw = 0.0, sf[3] = {0.0, 0.0, 0.0};
+    for_each_neighbour<D>(s, s.cell_start, S.x, xi, c, -1, [&](int j, double d2, const double*) {
+        if (S.kind && S.kind[j] != KIND_FLUID) return;
+        const double W = SmoothKernelFn<double, K>::value(sqrt(d2), s.h, s.sigma);
+        const double wj = W * S.m[j] / S.rho[j];
+        sw += wj;
+        for (int k = 0; k < w; ++k) sf[k] += wj * f[(size_t)j * w + k];
+    });
+    for (int k = 0; k < w; ++k) out[q * w + k] = sw > 0.0 ? sf[k] / sw : none_value;
+}
+
 // Kernel values on the device (checks of the device math).
 template <int K>
 __global__ void k_kernel_eval(int64_t n, const double* __restrict__ r, double* __restrict__ w,
@@ -1041,6 +1076,22 @@ hipError_t launch_pairs(const Bucket& b, int pairset, int32_t* counts, const int
         hipLaunchKernelGGL(k_pairs<2>, g, blk, 0, st, b.scenes, b.tiles, b.ntiles, pairset, counts, offs, out);
     else
         hipLaunchKernelGGL(k_pairs<3>, g, blk, 0, st, b.scenes, b.tiles, b.ntiles, pairset, counts, offs, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe(const Bucket& b, int64_t np, const double* pts, int field, double none_value, double* out,
+                        hipStream_t st) {
+    if (np <= 0) return hipSuccess;
+    dim3 g((unsigned)((np + kBlock - 1) / kBlock)), blk(kBlock);
+    if (b.dim == 2) {
+        ESPH_DISPATCH_K(b.kernel, {
+            hipLaunchKernelGGL((k_probe<2, K>), g, blk, 0, st, b.scenes, np, pts, field, none_value, out);
+        });
+    } else {
+        ESPH_DISPATCH_K(b.kernel, {
+            hipLaunchKernelGGL((k_probe<3, K>), g, blk, 0, st, b.scenes, np, pts, field, none_value, out);
+        });
+    }
     return hipGetLastError();
 }
 

@@ -283,6 +283,18 @@ class Scene:
                                                     out.shape[0], byref(m)))
         return out[: m.value]
 
+    def probe(self, points, field: int, none_value: float = float("nan")) -> np.ndarray:
+        """SPH interpolation of a fluid field at points (n, dim):
+        kernelValueInterpolation (DataIO/VTPToDataFrame.jl:77-142) on the
+        device; (n, dim) for FIELD_V, (n,) otherwise."""
+        P = _lib.f64(np.asarray(points, dtype=np.float64).reshape(-1, self.dim))
+        w = self.dim if field == _lib.FIELD_V else 1
+        out = np.empty(max(P.shape[0] * w, 1))
+        self.ctx.check(self.L.esph_scene_probe(self.handle, field, P.shape[0], _lib.dptr(P), none_value,
+                                               _lib.dptr(out)))
+        out = out[: P.shape[0] * w]
+        return out.reshape(-1, w) if w > 1 else out
+
     def enable_timing(self, on: bool = True) -> None:
         self.ctx.check(self.L.esph_scene_enable_timing(self.handle, int(on)))
 
@@ -362,11 +374,14 @@ def solve(*args, check=None, precision: str = "FP64", ctx=None, on_output=None, 
     (kernel, model, time_discretization[, vtp_io]) arguments.  ``check`` must
     be an ``AABBCheck`` (arbitrary closures cannot run on the device).
     ``on_output(step, time, state)`` is called at output steps with the state
-    read back (VTP writing itself is not part of this library).
+    read back.  With a VTPIO after the time discretisation, VTP files are
+    written as the reference does, by a host thread that overlaps the device
+    steps (vtp.AsyncVTPWriter).
     """
     k = next(i for i, a in enumerate(args) if isinstance(a, SmoothKernel_))
     groups = list(args[:k])
     kernel, model, td = args[k], args[k + 1], args[k + 2]
+    vtp_io = args[k + 3] if len(args) > k + 3 else None
     if check is not None and not isinstance(check, AABBCheck):
         raise TypeError("check must be an AABBCheck: device steps cannot call host closures")
     io = len(groups) == 4 and all(_is_fluid_group(g) for g in groups[:3])
@@ -381,23 +396,38 @@ def solve(*args, check=None, precision: str = "FP64", ctx=None, on_output=None, 
     desc = make_desc(fluid.dim, kernel, model, td, solver, PRECISIONS[precision], check,
                      planes=inlet_outlet if io else None, profile=user_defined_function if io else None)
     scene = Scene(desc, fluid, walls, ctx, buffers=tuple(bufs) if io else None)
+    writer = None
+    if vtp_io is not None:
+        from .vtp import AsyncVTPWriter
+
+        # writeVTP(0, 0.0, ...) then one file per output step, named by
+        # div(step, output_interval) (FluidWallSolve.jl:94-95,135-137)
+        writer = AsyncVTPWriter(vtp_io)
+        writer.submit(0, 0.0, [fluid] + bufs + walls)
     try:
         total = td.total_step_
         done = 0
         while done < total:
             n = total - done
-            if on_output is not None and td.output_interval_ > 0:
+            if (on_output is not None or writer is not None) and td.output_interval_ > 0:
                 n = min(n, td.output_interval_ - done % td.output_interval_)
             scene.step(n)
             done += n
-            if on_output is not None and isOutputStep(done, td):
-                on_output(done, done * td.dt_, scene.state())
+            if (on_output is not None or writer is not None) and isOutputStep(done, td):
+                st = scene.state()
+                if on_output is not None:
+                    on_output(done, done * td.dt_, st)
+                if writer is not None:
+                    bst = [scene.state(g) for g in (_lib.GROUP_INLET, _lib.GROUP_OUTLET)] if io else []
+                    writer.submit(done // td.output_interval_, done * td.dt_, [st] + bst + walls)
         scene.sync()
         out = scene.state()
         ids = scene.ids()
         bouts = [scene.state(g) for g in (_lib.GROUP_INLET, _lib.GROUP_OUTLET)] if io else []
     finally:
         scene.close()
+        if writer is not None:
+            writer.close()
     if io:
         # the groups change size: fluid gains inlet copies and loses leavers,
         # the outlet gains them; results replace the callers' contents

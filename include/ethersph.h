@@ -272,6 +272,14 @@ int esph_scene_get_group(esph_scene* scene, int32_t group, int32_t field, double
    sorted; same output contract as esph_neighbor_list. */
 int esph_scene_neighbours(esph_scene* scene, int32_t pairset, int32_t* out_pairs,
                           int64_t capacity, int64_t* m);
+/* kernelValueInterpolation (DataIO/VTPToDataFrame.jl:77-142) on the resident
+   fluid, FP64: at each of npoints probe points (npoints*dim doubles),
+   sum_j W(r) m_j / rho_j f_j / sum_j W(r) m_j / rho_j over the fluid particles
+   within R (pair rule above); none_value where there is none.  field is
+   ESPH_FIELD_V (dim values per point), _RHO, _P, _C or, for thermal solvers,
+   _T.  Synchronous. */
+int esph_scene_probe(esph_scene* scene, int32_t field, int64_t npoints, const double* points, double none_value,
+                     double* out);
 /* Average device time per step of each phase over the steps since the last
    call (milliseconds): [0] cell sort, [1] continuity+EOS, [2] forces+integrate,
    [3] Shepard (per step it ran), [4] 0 (the check runs inside the sort);

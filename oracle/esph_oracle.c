@@ -1058,3 +1058,35 @@ int64_t eo_scene_neighbours(eo_scene* s, int32_t pairset, int32_t* out_pairs, in
     if (s->nw == 0) return 0;
     return eo_neighbor_list(s->d.dim, s->k.influence_radius, s->n, s->x, s->nw, s->wx, 0, 1, out_pairs, cap);
 }
+
+/* kernelValueInterpolation (DataIO/VTPToDataFrame.jl:77-142) over the fluid
+   particles: brute force over every (probe, particle) pair in index order. */
+int eo_scene_probe(const eo_scene* s, int32_t field, int64_t np, const double* pts, double none_value,
+                   double* out) {
+    const int D = s->d.dim;
+    const double* f = NULL;
+    int w = 1;
+    switch (field) {
+        case ESPH_FIELD_V: f = s->v; w = D; break;
+        case ESPH_FIELD_RHO: f = s->rho; break;
+        case ESPH_FIELD_P: f = s->p; break;
+        case ESPH_FIELD_C: f = s->c; break;
+        case ESPH_FIELD_T: f = s->thermal ? s->t : NULL; break;
+        default: return -1;
+    }
+    if (!f) return -1;
+    const double R = s->k.influence_radius, R2 = R * R;
+    for (int64_t q = 0; q < np; ++q) {
+        double sw = 0.0, sf[3] = {0.0, 0.0, 0.0};
+        for (int64_t j = 0; j < s->n; ++j) {
+            const double d2 = dist2(D, pts + q * D, s->x + j * D);
+            if (!(d2 <= R2)) continue;
+            const double W = eo_kernel_value(&s->k, sqrt(d2));
+            const double wj = W * s->m[j] / s->rho[j];
+            sw += wj;
+            for (int k = 0; k < w; ++k) sf[k] += wj * f[j * w + k];
+        }
+        for (int k = 0; k < w; ++k) out[q * w + k] = sw > 0.0 ? sf[k] / sw : none_value;
+    }
+    return 0;
+}

@@ -60,6 +60,10 @@ int64_t eo_scene_fluid_count(const eo_scene* s);
 int64_t eo_scene_step_count(const eo_scene* s);
 /* Field in reference order; out holds n (or n*dim) doubles. */
 int eo_scene_get_fluid(const eo_scene* s, int32_t field, double* out);
+/* SPH interpolation of a fluid field (V: dim values per point) at np probe
+   points (VTPToDataFrame.jl:77-142); none_value where no particle is within R. */
+int eo_scene_probe(const eo_scene* s, int32_t field, int64_t np, const double* pts, double none_value,
+                   double* out);
 /* Pair set the next step would use; sorted; returns count (see above). */
 int64_t eo_scene_neighbours(eo_scene* s, int32_t pairset, int32_t* out_pairs, int64_t cap);
 

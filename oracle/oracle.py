@@ -49,6 +49,7 @@ def lib():
             "eo_scene_step_count": ([vp], c_int64),
             "eo_scene_get_fluid": ([vp, c_int32, DP], c_int32),
             "eo_scene_neighbours": ([vp, c_int32, POINTER(c_int32), c_int64], c_int64),
+            "eo_scene_probe": ([vp, c_int32, c_int64, DP, c_double, DP], c_int32),
         }
         for n, (a, r) in sig.items():
             getattr(L, n).argtypes = a
@@ -134,6 +135,14 @@ class OracleScene:
             assert self.L.eo_scene_get_group(self.h, group, field, _p(out)) == 0
         out = out[: n * w]
         return out.reshape(n, w) if w > 1 else out
+
+    def probe(self, points, field: int, none_value: float = float("nan")) -> np.ndarray:
+        P = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, self.dim))
+        w = self.dim if field == 1 else 1
+        out = np.empty(max(P.shape[0] * w, 1))
+        assert self.L.eo_scene_probe(self.h, field, P.shape[0], _p(P), none_value, _p(out)) == 0
+        out = out[: P.shape[0] * w]
+        return out.reshape(-1, w) if w > 1 else out
 
     def neighbours(self, pairset: int = 0) -> np.ndarray:
         m = self.L.eo_scene_neighbours(self.h, pairset, None, 0)
