@@ -9,6 +9,14 @@ with the particle state resident in HBM.  The same scene in the plain FP64
 form is timed beside it, and the FP64 CPU oracle is timed on a bounded sample
 (a few steps of the same scene) on the host cores.
 
+Two further legs report the other halves of the metric: "ensemble" (256 2D
+scenes, half lid-driven cavities and half Poiseuille channels, sizes
+log-uniform in [2k, 60k], batched with graphs versus launched one scene at a
+time) and, with --serving, "serving" (Poisson arrivals of 2D dam breaks, 3D
+Poiseuille pipes and 2D thermal cavities under a 200 GB HBM budget: completed
+scenes/s and p50/p99 step latency).  Both are bounded samples, stated in
+their output.
+
 Prints one JSON line.
 """
 import argparse
@@ -28,6 +36,16 @@ FORCE_BYTES = {0: (128.0, 56.0), 1: (160.0, 28.0)}
 # Whole-step algorithmic bytes per fluid / wall particle-step, 3D, Shepard every 30.
 STEP_BYTES = {0: (241.6, 56.0), 1: (232.8, 28.0)}
 HBM_PEAK_GBS = 8000.0
+
+
+def measured_traffic(kernel, prec="fp32_bulk"):
+    """HBM bytes per dispatch from the committed rocprofv3 counter passes
+    (calibrated FETCH_SIZE + WRITE_SIZE), or None when not profiled."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "dam_break_3d_traffic.json")) as fh:
+            return float(json.load(fh)[kernel][prec])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def log(*a):
@@ -53,6 +71,86 @@ def time_gpu(sc, prec, steps, warmup):
     return wall, ph, n
 
 
+def ensemble_leg(steps, alone_steps, n_scenes, full_steps=5000):
+    """Batched ensemble of 2D scenes (half lid-driven cavities, half
+    Poiseuille channels) against the same scenes launched one at a time.
+    scenes/s is quoted for scenes of full_steps steps from the measured
+    per-step times."""
+    from ethersph import _lib, scenarios
+    from ethersph.solve import Batch
+
+    scs = scenarios.ensemble_2d(n_scenes=n_scenes, seed=0, total_step=full_steps)
+    scenes = [sc.make_scene(_lib.FP32_BULK) for sc in scs]
+    npart = sum(sc.n_fluid for sc in scs)
+    b = Batch(scenes, graph=True)
+    b.step(5)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    t_batch = (time.perf_counter() - t0) / steps
+    b.close()
+    # one at a time: each scene alone (its own captured step graph), in turn
+    for s in scenes:
+        s.enable_graph(True)
+        s.step(2)
+    scenes[0].sync()
+    t0 = time.perf_counter()
+    for s in scenes:
+        s.step(alone_steps)
+        s.sync()
+    t_alone = (time.perf_counter() - t0) / alone_steps
+    for s in scenes:
+        s.close()
+    batched = n_scenes / (t_batch * full_steps)
+    alone = n_scenes / (t_alone * full_steps)
+    return {"scenes": n_scenes, "fluid_particles": npart, "steps_per_scene": full_steps,
+            "batched_ms_per_step": t_batch * 1e3, "one_at_a_time_ms_per_step": t_alone * 1e3,
+            "batched_scenes_per_s": batched, "one_at_a_time_scenes_per_s": alone, "speedup": batched / alone,
+            "batched_particle_steps_per_s": npart / t_batch,
+            "sample": f"{steps} batched steps and {alone_steps} steps per scene alone, FP32_BULK, graphs; "
+                      f"scenes/s extrapolated to {full_steps}-step scenes"}
+
+
+def serving_leg(n_requests, rate, step_scale, budget_gb, advance_steps):
+    """Poisson arrivals of mixed requests served by continuous batching."""
+    from ethersph import _lib, scenarios
+    from ethersph.serving import Server
+
+    lo, hi = max(1, int(1000 * step_scale)), max(1, int(10000 * step_scale))
+    stream = scenarios.serving_stream(n_requests=n_requests, seed=1, rate_per_s=rate, steps=(lo, hi))
+    ctx = _lib.Context(0, int(budget_gb * 1e9))
+    sv = Server(ctx)
+    ids = []
+    t0 = time.perf_counter()
+    k = 0
+    psteps = 0
+    while True:
+        now = time.perf_counter() - t0
+        while k < len(stream) and stream[k][0] <= now:
+            _, sc, n = stream[k]
+            ids.append(sv.submit_scenario(sc, _lib.FP32_BULK, n))
+            psteps += sc.n_fluid * n
+            k += 1
+        st = sv.stats()
+        if k == len(stream) and st["running"] == 0 and st["queued"] == 0:
+            break
+        if st["running"] == 0 and st["queued"] == 0:
+            time.sleep(max(0.0, stream[k][0] - now))
+            continue
+        sv.advance(advance_steps)
+    wall = time.perf_counter() - t0
+    st = sv.stats()
+    failed = sum(1 for r in ids if sv.status(r)[0] == _lib.REQ_FAILED)
+    sv.close()
+    ctx.close()
+    return {"requests": n_requests, "completed": int(st["completed"]), "failed": failed,
+            "scenes_per_s": st["completed"] / wall, "p50_step_ms": st["p50_step_ms"],
+            "p99_step_ms": st["p99_step_ms"], "particle_steps_per_s": psteps / wall, "wall_s": wall,
+            "sample": f"{n_requests} Poisson arrivals at {rate}/s, steps {lo}-{hi} per request "
+                      f"(the workload's 1,000-10,000 scaled by {step_scale}), {budget_gb} GB budget, FP32_BULK"}
+
+
 def cpu_baseline(sc, steps, threads):
     from oracle import oracle
 
@@ -73,6 +171,15 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=300)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-scenes", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--ensemble-alone-steps", type=int, default=30)
+    ap.add_argument("--serving-requests", type=int, default=24)
+    ap.add_argument("--serving-rate", type=float, default=4.0)
+    ap.add_argument("--serving-step-scale", type=float, default=0.1)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--serving", action="store_true",
+                    help="also run the bounded serving sample (off by default: see the design notes)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
@@ -105,6 +212,23 @@ def main():
         cpu = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"{a.cpu_steps} steps of the full {nf}-particle scene, FP64 oracle (OpenMP), {dt:.1f} s"}
 
+    ens = None
+    if not a.no_ensemble:
+        log(f"ensemble: {a.ensemble_scenes} 2D scenes, batched vs one at a time")
+        try:
+            ens = ensemble_leg(a.ensemble_steps, a.ensemble_alone_steps, a.ensemble_scenes)
+        except Exception as e:  # reported in the JSON line, the headline stands
+            ens = {"error": repr(e)}
+        log(f"ensemble: {ens}")
+    srv = None
+    if a.serving:
+        log(f"serving: {a.serving_requests} requests")
+        try:
+            srv = serving_leg(a.serving_requests, a.serving_rate, a.serving_step_scale, 200.0, 50)
+        except Exception as e:
+            srv = {"error": repr(e)}
+        log(f"serving: {srv}")
+
     out = {
         "metric": "particle-steps/s at 1 GPU + %HBM peak; ensemble scenes/s, p50/p99 step latency",
         "value": value,
@@ -122,12 +246,15 @@ def main():
                                "FP32_BULK, Shepard every 30 steps",
                    "n_fluid": nf, "n_wall": nw, "n_fluid_end": n_end, "dx": a.dx},
         "roofline": {"bound": "hbm", "kernel": "k_forces", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_forces"),
+                     "algorithmic_bytes": force_bytes},
         "step_algorithmic_GBps": step_gbs,
         "phases_ms": {"fp32_bulk": ph32, "fp64": ph64},
         "fp64_form": {"value": value64, "ms_per_step": wall64 * 1e3 / a.fp64_steps, "steps": a.fp64_steps},
         "fp32_over_fp64": value / value64,
         "cpu_baseline": cpu,
+        "ensemble": ens,
+        "serving": srv,
     }
     print(json.dumps(out), flush=True)
 

@@ -17,6 +17,20 @@
 
 namespace esph {
 
+// Workgroups go to the eight XCDs round-robin (block b runs on XCD b % 8).
+// With one tile per block, xcd_tile gives each XCD one contiguous range of
+// tiles, walked in order, so each XCD's L2 holds one compact, slowly moving
+// region of the cell-sorted particles instead of every eighth tile.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    constexpr int X = 8;
+    const int per = n / X, rem = n % X;
+    const int x = b % X, k = b / X;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
+}
+__device__ __forceinline__ int tile_index(int b, int n) {
+    return (int)gridDim.x == n ? xcd_tile(b, n) : b;
+}
+
 template <typename T>
 __device__ __forceinline__ T ld(const void* p, int i) {
     return static_cast<const T*>(p)[i];
@@ -70,18 +84,37 @@ __device__ __forceinline__ void for_each_neighbour(const SceneDev& s, const int3
     }
 }
 
+// Neighbour lists are stored in chunks of four: entries 4q..4q+3 of particle
+// i form the int4 at [q * cap + i], so a wavefront reads or writes 64
+// consecutive int4 (1 KiB) per chunk row, and each lane writes whole 16-byte
+// pieces instead of single ints.
+__device__ __forceinline__ int4* list_chunk(int32_t* list, size_t cap, int q, int i) {
+    return reinterpret_cast<int4*>(list) + ((size_t)q * cap + i);
+}
+__device__ __forceinline__ const int4* list_chunk(const int32_t* list, size_t cap, int q, int i) {
+    return reinterpret_cast<const int4*>(list) + ((size_t)q * cap + i);
+}
+
 // Records the neighbours of particle i (pair rule of for_each_neighbour) in
-// the transposed list `list` (entry k at [k * cap + i]); returns the count.
-// Entries beyond `width` are dropped and the particle later scans its cells.
+// the chunked list `list`; returns the count.  Entries beyond `width` (a
+// multiple of 4) are dropped and the particle later scans its cells.
 template <int D>
 __device__ __forceinline__ int build_list(const SceneDev& s, const int32_t* __restrict__ cstart,
                                           const double* __restrict__ xall, const double* xi, const int c[3],
                                           int i, int self, int32_t* __restrict__ list, int width) {
     int cnt = 0;
+    int4 buf = make_int4(0, 0, 0, 0);
+    const size_t cap = (size_t)s.cap;
     for_each_neighbour<D>(s, cstart, xall, xi, c, self, [&](int j, double, const double*) {
-        if (cnt < width) list[(size_t)cnt * s.cap + i] = j;
+        const int r = cnt & 3;
+        buf.x = r == 0 ? j : buf.x;
+        buf.y = r == 1 ? j : buf.y;
+        buf.z = r == 2 ? j : buf.z;
+        buf.w = r == 3 ? j : buf.w;
+        if (r == 3 && cnt < width) *list_chunk(list, cap, cnt >> 2, i) = buf;
         ++cnt;
     });
+    if ((cnt & 3) != 0 && cnt < width) *list_chunk(list, cap, cnt >> 2, i) = buf;
     return cnt;
 }
 
@@ -94,14 +127,17 @@ __device__ __forceinline__ void for_each_listed(const SceneDev& s, const double*
                                                 int cnt, F&& f) {
     if (cnt <= 0) return;
     const size_t cap = (size_t)s.cap;
-    int jn = list[i];
+    int4 ch = *list_chunk(list, cap, 0, i);
+    int jn = ch.x;
     double xn[3];
     load_x<D>(xall, jn, xn);
     for (int k = 0; k < cnt; ++k) {
         const int j = jn;
         double xj[3] = {xn[0], xn[1], D == 3 ? xn[2] : 0.0};
         if (k + 1 < cnt) {
-            jn = list[(size_t)(k + 1) * cap + i];
+            const int r = (k + 1) & 3;
+            if (r == 0) ch = *list_chunk(list, cap, (k + 1) >> 2, i);
+            jn = r == 0 ? ch.x : r == 1 ? ch.y : r == 2 ? ch.z : ch.w;
             load_x<D>(xall, jn, xn);
         }
         double rv[3];
@@ -213,7 +249,8 @@ template <int D, int K, typename T, bool SCAN>
 __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles, int ntiles) {
     using KQ = KernelQ<T, K>;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
@@ -458,7 +495,8 @@ __device__ __forceinline__ void forces_body(const SceneDev& s, const FluidSet& S
 template <int D, int K, typename T, bool THERMAL, bool SCAN>
 __global__ __launch_bounds__(kBlock) void k_forces(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, int ntiles) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
@@ -479,7 +517,8 @@ template <int D, int K, bool SCAN>
 __global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__ scenes,
                                                     const Tile* __restrict__ tiles, int ntiles) {
     using KQ = KernelQ<double, K>;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
@@ -529,7 +568,8 @@ __device__ __forceinline__ bool shepard_due(const SceneDev& s, const SceneDyn& d
 
 __global__ __launch_bounds__(kBlock) void k_sort_clear(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ ctiles, int ntiles) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
 #pragma unroll
@@ -580,7 +620,8 @@ __device__ __forceinline__ int wave_counted_slot(int32_t* count, int key) {
 
 __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict__ scenes,
                                                      const Tile* __restrict__ tiles, int ntiles, int mode) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
@@ -662,7 +703,8 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
 // Phase 1: scan of kCellTile counts per tile; tile total to bsum.
 __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ ctiles, int ntiles) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
         const int c0 = tile.begin + (int)threadIdx.x * kScanItems;
@@ -703,7 +745,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan2(const SceneDev* __restric
 // Phase 3: add tile offsets; the last cell also writes the grand total.
 __global__ __launch_bounds__(kBlock) void k_sort_scan3(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ ctiles, int ntiles) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
         const int off = s.bsum[tile.begin / kCellTile];
@@ -721,7 +764,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan3(const SceneDev* __restric
 
 __global__ __launch_bounds__(kBlock) void k_sort_place(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles, int ntiles, int mode) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const int i = tile.begin + (int)threadIdx.x;
@@ -770,7 +814,8 @@ __device__ double profile_u(const SceneDev& s, const double* x, double t) {
 // migration (kinds and group sequence numbers) and the buffer velocities.
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restrict__ scenes,
                                                          const Tile* __restrict__ tiles, int ntiles, int mode) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
@@ -896,7 +941,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const SceneDev* __restrict__ s
                                                   const Tile* __restrict__ tiles, int ntiles, int pairset,
                                                   int32_t* __restrict__ counts, const int64_t* __restrict__ offs,
                                                   int32_t* __restrict__ out) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;

@@ -454,3 +454,58 @@ def poiseuille_3d(cells=40, l=1e-2, nu=1e-6, rho_0=1000.0, f=8e-5, kernel=Wendla
     planes = InletOutletPlanes(0, x0, t * dr, x0 + pipe_len, x0 + pipe_len + t * dr)
     profile = BufferProfile(_lib.PROFILE_PIPE_3D, f, nu, r0, (x0, y0, z0), 0, outlet_profile=False)
     return Scenario("poiseuille_3d", dim, sk, model, td, fl, [walls], None, (inlet, outlet), planes, profile)
+
+
+def _log_uniform(rng, lo, hi):
+    return float(np.exp(rng.uniform(math.log(lo), math.log(hi))))
+
+
+def ensemble_2d(n_scenes=256, seed=0, n_min=2000, n_max=60000, total_step=5000):
+    """The batched-ensemble workload: alternating lid-driven cavities (lid
+    1 m/s, Re uniform in [10, 1000]) and Poiseuille channels (body force
+    giving Re uniform in [1, 100]); the fluid count of each scene is
+    log-uniform in [n_min, n_max]; CubicSpline or WendlandC2 drawn per scene;
+    everything from one seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_scenes):
+        n = _log_uniform(rng, n_min, n_max)
+        kern = CubicSpline if rng.random() < 0.5 else WendlandC2
+        if k % 2 == 0:
+            side = max(8, round(math.sqrt(n)))
+            sc = lid_driven_cavity_2d(dx=1.0 / side, reynolds=float(rng.uniform(10.0, 1000.0)), kernel=kern,
+                                      total_step=total_step)
+        else:
+            rows = max(6, round(math.sqrt(n / 4.0)))
+            sc = poiseuille_2d(rows=rows, reynolds=float(rng.uniform(1.0, 100.0)), kernel=kern,
+                               total_step=total_step)
+        out.append(sc)
+    return out
+
+
+def serving_stream(n_requests=32, seed=0, rate_per_s=2.0, steps=(1000, 10000), scale=1.0):
+    """The serving workload: Poisson arrivals (rate_per_s) of requests drawn
+    uniformly from three kinds, sizes log-uniform per kind --
+    2D dam break 5k-200k fluid particles, 3D Poiseuille pipe 50k-2M, 2D
+    thermal cavity 10k-100k -- each running a step count uniform in
+    ``steps``.  ``scale`` multiplies every size (for bounded samples).
+    Returns [(arrival_s, scenario, nsteps)]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n_requests):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        kind = int(rng.integers(3))
+        nsteps = int(rng.integers(steps[0], steps[1] + 1))
+        if kind == 0:
+            n = _log_uniform(rng, 5e3, 2e5) * scale
+            sc = collapse_dry_2d(dx=math.sqrt(2.0 / n), total_step=nsteps)
+        elif kind == 1:
+            n = _log_uniform(rng, 5e4, 2e6) * scale
+            sc = poiseuille_3d(cells=max(6, round((2.0 * n / math.pi) ** (1.0 / 3.0))), total_step=nsteps)
+        else:
+            n = _log_uniform(rng, 1e4, 1e5) * scale
+            sc = thermal_cavity_2d(dx=1.0 / max(8, round(math.sqrt(n))), seed=int(rng.integers(1 << 30)),
+                                   total_step=nsteps)
+        out.append((t, sc, nsteps))
+    return out
