@@ -391,9 +391,9 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         const double a = d.influence_radius / spacing;
         const double est = D == 2 ? 3.14159265358979 * a * a : 4.18879020478639 * a * a * a;
         int k = (int)std::ceil(1.5 * est + 16.0);
-        k = std::min(std::max((k + 7) / 8 * 8, 16), 1024);
+        k = std::min(std::max((k + 15) / 16 * 16, 16), 1024);  // whole 16-entry chunks
         hd.kff = k;
-        hd.kfw = nw > 0 ? std::min(std::max((k / 2 + 7) / 8 * 8, 16), 1024) : 0;
+        hd.kfw = nw > 0 ? std::min(std::max((k / 2 + 15) / 16 * 16, 16), 1024) : 0;
     }
 
     // size check against the context budget
@@ -406,7 +406,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
                    (size_t)nw * (3 * D * G + 3 * G + 2 * I + 9 * P) + (size_t)(hd.ncell + 1) * I +
                    sizeof(SceneDyn) + sizeof(SceneDev) +
                    ((size_t)(cap + kBlock - 1) / kBlock + (size_t)hd.cell_tiles) * sizeof(Tile) +
-                   (size_t)cap * I * (hd.kff + hd.kfw + 2);
+                   (size_t)cap * I * (hd.kff + hd.kfw + 2) + 2 * (size_t)cap * 4 * F;
     if (ctx->budget && ctx->used + bytes > ctx->budget)
         return fail(ctx, ESPH_ERR_BUDGET, "scene needs " + std::to_string(bytes) + " bytes; budget left " +
                                               std::to_string(ctx->budget - ctx->used));
@@ -433,6 +433,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         S.A = dal(4 * P * cap);
         S.B = dal(4 * P * cap);
         S.kap = (thermal && D == 3) ? dal(P * cap) : nullptr;
+        S.xf = dal(F * (D == 3 ? 4 : 2) * cap);
     }
     hd.xn = (double*)dal(G * cap * D);
     hd.vn = (double*)dal(G * cap * D);
@@ -472,6 +473,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         hd.wA = dal(4 * P * std::max(nw, 1));
         hd.wB = dal(4 * P * std::max(nw, 1));
         hd.wkap = dal(P * std::max(nw, 1));
+        hd.wxf = dal(F * (D == 3 ? 4 : 2) * std::max(nw, 1));
     }
     hd.dyn = (SceneDyn*)dal(sizeof(SceneDyn));
     s->dev = (SceneDev*)dal(sizeof(SceneDev));
@@ -521,6 +523,14 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         up(hd.wkind, ws.kind.data(), I * nw);
         up(hd.wid, ws.wid.data(), I * nw);
         up(hd.wcell_start, ws.wcs.data(), I * (hd.ncell + 1));
+        // grid-relative float positions (the neighbour-list prefilter)
+        {
+            const int W = D == 3 ? 4 : 2;
+            ws.fX.assign((size_t)W * std::max(nw, 1), 0.0f);
+            for (int p = 0; p < nw; ++p)
+                for (int k = 0; k < D; ++k) ws.fX[(size_t)W * p + k] = (float)(ws.x[(size_t)p * D + k] - hd.lo[k]);
+            up(hd.wxf, ws.fX.data(), F * ws.fX.size());
+        }
         // packed wall records (see SceneDev::wA / wB / wkap)
         {
             std::vector<double> A(4 * (size_t)std::max(nw, 1), 0.0), B(4 * (size_t)std::max(nw, 1), 0.0);

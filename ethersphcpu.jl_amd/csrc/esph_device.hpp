@@ -65,6 +65,10 @@ struct FluidSet {
     void* A;
     void* B;
     void* kap;
+    // positions relative to the grid origin in float (float4 in 3D, float2 in
+    // 2D): the neighbour-list build tests candidates on these and falls back
+    // to the FP64 rule only inside a rounding band around R
+    void* xf;
 };
 
 struct SceneDev {
@@ -104,7 +108,8 @@ struct SceneDev {
     int32_t kff;          // neighbour-list width (fluid-fluid)
     int32_t kfw;          // neighbour-list width (fluid-wall)
     int32_t pad0;
-    // per-step neighbour lists, transposed: entry k of particle i at [k * cap + i]
+    // per-step neighbour lists in 4-entry chunks: entry k of particle i at
+    // [((k / 4) * cap + i) * 4 + k % 4]
     int32_t* nbr;
     int32_t* nnb;         // fluid-fluid count per particle
     int32_t* wnbr;
@@ -126,6 +131,7 @@ struct SceneDev {
     void* wA;
     void* wB;
     void* wkap;
+    void* wxf;       // wall positions relative to lo in float (float4 / float2)
     // inlet/outlet solver: migration planes along io_axis and buffer profile
     int32_t io_axis, profile, profile_terms, outlet_profile;
     double inlet_enter, inlet_shift, outlet_enter, outlet_exit;

@@ -85,36 +85,100 @@ __device__ __forceinline__ void for_each_neighbour(const SceneDev& s, const int3
 }
 
 // Neighbour lists are stored in chunks of four: entries 4q..4q+3 of particle
-// i form the int4 at [q * cap + i], so a wavefront reads or writes 64
-// consecutive int4 (1 KiB) per chunk row, and each lane writes whole 16-byte
-// pieces instead of single ints.
-__device__ __forceinline__ int4* list_chunk(int32_t* list, size_t cap, int q, int i) {
-    return reinterpret_cast<int4*>(list) + ((size_t)q * cap + i);
-}
-__device__ __forceinline__ const int4* list_chunk(const int32_t* list, size_t cap, int q, int i) {
-    return reinterpret_cast<const int4*>(list) + ((size_t)q * cap + i);
+// i are the int4 at [q * cap + i], so a wavefront reads 64 consecutive int4
+// (1 KiB) per chunk row and each lane writes whole 16-byte pieces.
+constexpr int kChunk = 4;
+__device__ __forceinline__ size_t list_at(size_t cap, int k, int i) {
+    return (((size_t)(k >> 2) * cap + (size_t)i) << 2) + (size_t)(k & 3);
 }
 
-// Records the neighbours of particle i (pair rule of for_each_neighbour) in
-// the chunked list `list`; returns the count.  Entries beyond `width` (a
-// multiple of 4) are dropped and the particle later scans its cells.
+// Candidates of particle i over the adjacent cells, tested first in float on
+// the grid-relative positions xf: a candidate whose float distance is clearly
+// inside or clearly outside R is decided there; only those within the
+// rounding band tau of R take the FP64 rule of for_each_neighbour.  tau bounds
+// the float error: 2^-21 (|xr_i|_inf + 2 R), so the decisions equal the FP64
+// rule's.  Calls f(j) for every neighbour, in cell order.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_candidate_f32(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                                       const void* __restrict__ xfall,
+                                                       const double* __restrict__ xall, const double* xi,
+                                                       const int c[3], int self, F&& f) {
+    float xr[3] = {(float)(xi[0] - s.lo[0]), (float)(xi[1] - s.lo[1]), D == 3 ? (float)(xi[2] - s.lo[2]) : 0.0f};
+    const double amax = fmax(fabs((double)xr[0]), fmax(fabs((double)xr[1]), fabs((double)xr[2])));
+    const double tau = ldexp(amax + 2.0 * s.R, -21);
+    const double rin = s.R - tau, rout = s.R + tau;
+    const float lo2 = rin > 0.0 ? (float)(rin * rin) * (1.0f - 0x1p-22f) : -1.0f;
+    const float hi2 = (float)(rout * rout) * (1.0f + 0x1p-22f);
+    const double R2 = s.R2;
+    const int zlo = D == 3 ? -1 : 0, zhi = D == 3 ? 1 : 0;
+    const int cx0 = c[0] > 0 ? c[0] - 1 : 0;
+    const int cx1 = c[0] + 1 < s.nc[0] ? c[0] + 1 : s.nc[0] - 1;
+    for (int dz = zlo; dz <= zhi; ++dz) {
+        const int cz = c[2] + dz;
+        if (cz < 0 || cz >= s.nc[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int cy = c[1] + dy;
+            if (cy < 0 || cy >= s.nc[1]) continue;
+            const int j0 = cstart[cell_linear(s, cx0, cy, cz)];
+            const int j1 = cstart[cell_linear(s, cx1, cy, cz) + 1];
+            for (int j = j0; j < j1; j += 4) {
+                float d2f[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = j + u < j1 ? j + u : j1 - 1;
+                    float dx, dy_, dz_ = 0.0f;
+                    if (D == 3) {
+                        const float4 q = static_cast<const float4*>(xfall)[jj];
+                        dx = xr[0] - q.x;
+                        dy_ = xr[1] - q.y;
+                        dz_ = xr[2] - q.z;
+                    } else {
+                        const float2 q = static_cast<const float2*>(xfall)[jj];
+                        dx = xr[0] - q.x;
+                        dy_ = xr[1] - q.y;
+                    }
+                    d2f[u] = dx * dx + dy_ * dy_ + dz_ * dz_;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = j + u;
+                    if (jj >= j1 || jj == self || !(d2f[u] <= hi2)) continue;
+                    bool in = d2f[u] < lo2;
+                    if (!in) {  // rounding band: the FP64 rule decides
+                        double xj[3];
+                        load_x<D>(xall, jj, xj);
+                        double rv[3];
+                        in = dist2_exact<D>(xi, xj, rv) <= R2;
+                    }
+                    if (in) f(jj);
+                }
+            }
+        }
+    }
+}
+
+// Records the neighbours of particle i in the chunked list `list`; returns
+// the count.  Entries fill a 4-entry register buffer flushed as one int4.
+// Entries beyond `width` (a multiple of 4) are dropped and the particle
+// later scans its cells.
 template <int D>
 __device__ __forceinline__ int build_list(const SceneDev& s, const int32_t* __restrict__ cstart,
-                                          const double* __restrict__ xall, const double* xi, const int c[3],
-                                          int i, int self, int32_t* __restrict__ list, int width) {
+                                          const void* __restrict__ xfall, const double* __restrict__ xall,
+                                          const double* xi, const int c[3], int i, int self,
+                                          int32_t* __restrict__ list, int width) {
     int cnt = 0;
     int4 buf = make_int4(0, 0, 0, 0);
     const size_t cap = (size_t)s.cap;
-    for_each_neighbour<D>(s, cstart, xall, xi, c, self, [&](int j, double, const double*) {
+    for_each_candidate_f32<D>(s, cstart, xfall, xall, xi, c, self, [&](int j) {
         const int r = cnt & 3;
         buf.x = r == 0 ? j : buf.x;
         buf.y = r == 1 ? j : buf.y;
         buf.z = r == 2 ? j : buf.z;
         buf.w = r == 3 ? j : buf.w;
-        if (r == 3 && cnt < width) *list_chunk(list, cap, cnt >> 2, i) = buf;
+        if (r == 3 && cnt < width) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
         ++cnt;
     });
-    if ((cnt & 3) != 0 && cnt < width) *list_chunk(list, cap, cnt >> 2, i) = buf;
+    if ((cnt & 3) != 0 && cnt < width) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
     return cnt;
 }
 
@@ -127,7 +191,7 @@ __device__ __forceinline__ void for_each_listed(const SceneDev& s, const double*
                                                 int cnt, F&& f) {
     if (cnt <= 0) return;
     const size_t cap = (size_t)s.cap;
-    int4 ch = *list_chunk(list, cap, 0, i);
+    int4 ch = *reinterpret_cast<const int4*>(list + list_at(cap, 0, i));
     int jn = ch.x;
     double xn[3];
     load_x<D>(xall, jn, xn);
@@ -136,7 +200,7 @@ __device__ __forceinline__ void for_each_listed(const SceneDev& s, const double*
         double xj[3] = {xn[0], xn[1], D == 3 ? xn[2] : 0.0};
         if (k + 1 < cnt) {
             const int r = (k + 1) & 3;
-            if (r == 0) ch = *list_chunk(list, cap, (k + 1) >> 2, i);
+            if (r == 0) ch = *reinterpret_cast<const int4*>(list + list_at(cap, k + 1, i));
             jn = r == 0 ? ch.x : r == 1 ? ch.y : r == 2 ? ch.z : ch.w;
             load_x<D>(xall, jn, xn);
         }
@@ -272,11 +336,11 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
         if (!SCAN) {
             int c[3];
             cell_of<D>(s, xi, c);
-            cnt = build_list<D>(s, s.cell_start, S.x, xi, c, i, i, s.nbr, s.kff);
+            cnt = build_list<D>(s, s.cell_start, S.xf, S.x, xi, c, i, i, s.nbr, s.kff);
             s.nnb[i] = cnt;
             bool over = cnt > s.kff;
             if (s.nw > 0) {
-                const int cw = build_list<D>(s, s.wcell_start, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
+                const int cw = build_list<D>(s, s.wcell_start, s.wxf, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
                 s.nnbw[i] = cw;
                 over = over || cw > s.kfw;
             }
@@ -876,6 +940,11 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
             O.x[p * D + k] = x[k];
             O.v[p * D + k] = v[k];
         }
+        if (D == 3)
+            static_cast<float4*>(O.xf)[p] = make_float4((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]),
+                                                        (float)(x[2] - s.lo[2]), 0.0f);
+        else
+            static_cast<float2*>(O.xf)[p] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
         const bool rho_new = mode == 1 || (shepard_due(s, dyn) && src_fluid && !spawn);
         const double rho = rho_new ? s.rhon[src] : S.rho[src];
         const double pp = S.p[src];

@@ -40,7 +40,7 @@ struct WallHost {
 struct WallStage {
     std::vector<double> x, n_, v, gap, T, kappa;
     std::vector<int32_t> kind, wid, wcs;
-    std::vector<float> fA, fB, fK;
+    std::vector<float> fA, fB, fK, fX;
     std::vector<double> dA, dB, dK;
 };
 
