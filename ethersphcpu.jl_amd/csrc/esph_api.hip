@@ -285,7 +285,10 @@ static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) 
     }
     for (int k = 0; k < D; ++k)
         if (!(lo[k] <= hi[k])) { lo[k] = 0.0; hi[k] = 0.0; }
-    double cs = s->d.influence_radius * 1.0001;
+    // cells half of R wide (a 5-cell stencil per axis: about 300 candidates per
+    // particle in 3D instead of about 500 with R-wide cells), widened when the
+    // grid would exceed 2^27 cells
+    double cs = s->d.influence_radius * 1.0001 / 2.0;
     for (;;) {
         int64_t ncell = 1;
         for (int k = 0; k < 3; ++k) {
@@ -299,6 +302,9 @@ static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) 
         cs *= 1.25;  // wider cells keep the search exact
     }
     hd.inv_cs = 1.0 / cs;
+    hd.cs = cs;
+    hd.stencil = (int32_t)std::ceil(s->d.influence_radius / cs);
+    if ((double)hd.stencil * cs <= s->d.influence_radius) hd.stencil += 1;
 }
 
 static int cell_of_host(const SceneDev& hd, int D, const double* x) {
@@ -1026,16 +1032,18 @@ extern "C" int esph_scene_phase_times(esph_scene* s, double* ms5, int64_t* steps
     return ESPH_OK;
 }
 
-extern "C" int esph_scene_stats(esph_scene* s, int64_t* out4) {
-    if (!s || !out4) return ESPH_ERR_ARG;
+extern "C" int esph_scene_stats(esph_scene* s, int64_t* out6) {
+    if (!s || !out6) return ESPH_ERR_ARG;
     if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
     SceneDyn dyn;
     int rc = read_dyn(s, &dyn);
     if (rc) return rc;
-    out4[0] = dyn.overflow;
-    out4[1] = s->h.kff;
-    out4[2] = s->h.kfw;
-    out4[3] = s->h.ncell;
+    out6[0] = dyn.overflow;
+    out6[1] = s->h.kff;
+    out6[2] = s->h.kfw;
+    out6[3] = s->h.ncell;
+    out6[4] = dyn.outside;
+    out6[5] = dyn.err;
     return ESPH_OK;
 }
 

@@ -39,7 +39,7 @@ struct SceneDyn {
     int32_t nleave;    // inlet/outlet: fluid particles entering the outlet this step
     int32_t next_fluid_id;   // next sequence number of the fluid group
     int32_t next_outlet_id;  // next sequence number of the outlet group
-    int32_t pad;
+    int32_t outside;   // particles more than one cell outside the grid at the last sort
 };
 
 // One sorted copy of the fluid state.  Masters are binary64 SoA arrays (state
@@ -89,9 +89,13 @@ struct SceneDev {
     double rho0, c0, p0, b, nu0, t0, cp, beta, dt;
     double g[3];
     double check_min[3], check_max[3];
-    // cell grid
+    // cell grid: cells cs wide (cs >= R / stencil); neighbours of a cell lie
+    // within `stencil` cells on each axis
     double lo[3];
     double inv_cs;
+    double cs;
+    int32_t stencil;
+    int32_t pad1;
     // fluid
     FluidSet set[2];
     double* xn;      // step outputs in the current set's order

@@ -49,39 +49,81 @@ __device__ __forceinline__ void load_x(const double* __restrict__ x, int j, doub
     }
 }
 
-// Iterates the particles of cells adjacent to cell c (3 x-contiguous ranges
-// in 2D, 9 in 3D) whose distance to xi satisfies the neighbour rule.  The
-// candidate loop issues the loads of four candidates before testing them.
+// Candidate rows of a point xi in cell c: the grid's cells are at least R /
+// stencil wide, so neighbours lie within `stencil` cells on every axis.  For
+// each (dy, dz) the cells of the row are x-contiguous in memory, so a row is
+// one index range [j0, j1).  Rows and row ends whose nearest point is
+// farther than R from xi (judged from xi's offset inside its cell, with a
+// slack that keeps the test conservative under rounding and clamping) are
+// skipped.  Calls row(j0, j1) for each remaining row.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_row(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                             const double* xi, const int c[3], F&& row) {
+    const int m = s.stencil;
+    const double cs = s.cs;
+    const double slack = 1e-7 * cs;
+    const double lim = s.R2 * (1.0 + 1e-6);
+    double p[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p[k] = k < D ? xi[k] - (s.lo[k] + (double)c[k] * cs) : 0.0;
+    // distance from xi to the slab of the cell at offset o along one axis
+    auto dmin = [&](int o, double pk) {
+        double d = o == 0 ? 0.0 : (o > 0 ? (double)(o - 1) * cs + (cs - pk) : (double)(-o - 1) * cs + pk);
+        d -= slack;
+        return d > 0.0 ? d : 0.0;
+    };
+    const int mz = D == 3 ? m : 0;
+    for (int dz = -mz; dz <= mz; ++dz) {
+        const int cz = c[2] + dz;
+        if (cz < 0 || cz >= s.nc[2]) continue;
+        const double rz = D == 3 ? dmin(dz, p[2]) : 0.0;
+        for (int dy = -m; dy <= m; ++dy) {
+            const int cy = c[1] + dy;
+            if (cy < 0 || cy >= s.nc[1]) continue;
+            const double ry = dmin(dy, p[1]);
+            const double r2 = ry * ry + rz * rz;
+            if (r2 > lim) continue;
+            int xl = -m, xh = m;
+            while (xl < 0) {
+                const double d = dmin(xl, p[0]);
+                if (d * d + r2 <= lim) break;
+                ++xl;
+            }
+            while (xh > 0) {
+                const double d = dmin(xh, p[0]);
+                if (d * d + r2 <= lim) break;
+                --xh;
+            }
+            const int cx0 = c[0] + xl > 0 ? c[0] + xl : 0;
+            const int cx1 = c[0] + xh < s.nc[0] - 1 ? c[0] + xh : s.nc[0] - 1;
+            if (cx0 > cx1) continue;
+            row(cstart[cell_linear(s, cx0, cy, cz)], cstart[cell_linear(s, cx1, cy, cz) + 1]);
+        }
+    }
+}
+
+// Iterates the particles of the candidate rows of xi whose distance
+// satisfies the neighbour rule (FP64, exact).  The candidate loop issues the
+// loads of four candidates before testing them.
 template <int D, typename F>
 __device__ __forceinline__ void for_each_neighbour(const SceneDev& s, const int32_t* __restrict__ cstart,
                                                    const double* __restrict__ xall, const double* xi,
                                                    const int c[3], int self, F&& f) {
-    const int zlo = D == 3 ? -1 : 0, zhi = D == 3 ? 1 : 0;
-    const int cx0 = c[0] > 0 ? c[0] - 1 : 0;
-    const int cx1 = c[0] + 1 < s.nc[0] ? c[0] + 1 : s.nc[0] - 1;
     const double R2 = s.R2;
-    for (int dz = zlo; dz <= zhi; ++dz) {
-        const int cz = c[2] + dz;
-        if (cz < 0 || cz >= s.nc[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int cy = c[1] + dy;
-            if (cy < 0 || cy >= s.nc[1]) continue;
-            const int j0 = cstart[cell_linear(s, cx0, cy, cz)];
-            const int j1 = cstart[cell_linear(s, cx1, cy, cz) + 1];
-            for (int j = j0; j < j1; j += 4) {
-                double xj[4][3];
+    for_each_row<D>(s, cstart, xi, c, [&](int j0, int j1) {
+        for (int j = j0; j < j1; j += 4) {
+            double xj[4][3];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) load_x<D>(xall, j + u < j1 ? j + u : j1 - 1, xj[u]);
+            for (int u = 0; u < 4; ++u) load_x<D>(xall, j + u < j1 ? j + u : j1 - 1, xj[u]);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int jj = j + u;
-                    double rv[3];
-                    const double d2 = dist2_exact<D>(xi, xj[u], rv);
-                    if (jj < j1 && jj != self && d2 <= R2) f(jj, d2, rv);
-                }
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j + u;
+                double rv[3];
+                const double d2 = dist2_exact<D>(xi, xj[u], rv);
+                if (jj < j1 && jj != self && d2 <= R2) f(jj, d2, rv);
             }
         }
-    }
+    });
 }
 
 // Neighbour lists are stored in chunks of four: entries 4q..4q+3 of particle
@@ -110,51 +152,40 @@ __device__ __forceinline__ void for_each_candidate_f32(const SceneDev& s, const 
     const float lo2 = rin > 0.0 ? (float)(rin * rin) * (1.0f - 0x1p-22f) : -1.0f;
     const float hi2 = (float)(rout * rout) * (1.0f + 0x1p-22f);
     const double R2 = s.R2;
-    const int zlo = D == 3 ? -1 : 0, zhi = D == 3 ? 1 : 0;
-    const int cx0 = c[0] > 0 ? c[0] - 1 : 0;
-    const int cx1 = c[0] + 1 < s.nc[0] ? c[0] + 1 : s.nc[0] - 1;
-    for (int dz = zlo; dz <= zhi; ++dz) {
-        const int cz = c[2] + dz;
-        if (cz < 0 || cz >= s.nc[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int cy = c[1] + dy;
-            if (cy < 0 || cy >= s.nc[1]) continue;
-            const int j0 = cstart[cell_linear(s, cx0, cy, cz)];
-            const int j1 = cstart[cell_linear(s, cx1, cy, cz) + 1];
-            for (int j = j0; j < j1; j += 4) {
-                float d2f[4];
+    for_each_row<D>(s, cstart, xi, c, [&](int j0, int j1) {
+        for (int j = j0; j < j1; j += 4) {
+            float d2f[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int jj = j + u < j1 ? j + u : j1 - 1;
-                    float dx, dy_, dz_ = 0.0f;
-                    if (D == 3) {
-                        const float4 q = static_cast<const float4*>(xfall)[jj];
-                        dx = xr[0] - q.x;
-                        dy_ = xr[1] - q.y;
-                        dz_ = xr[2] - q.z;
-                    } else {
-                        const float2 q = static_cast<const float2*>(xfall)[jj];
-                        dx = xr[0] - q.x;
-                        dy_ = xr[1] - q.y;
-                    }
-                    d2f[u] = dx * dx + dy_ * dy_ + dz_ * dz_;
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j + u < j1 ? j + u : j1 - 1;
+                float dx, dy_, dz_ = 0.0f;
+                if (D == 3) {
+                    const float4 q = static_cast<const float4*>(xfall)[jj];
+                    dx = xr[0] - q.x;
+                    dy_ = xr[1] - q.y;
+                    dz_ = xr[2] - q.z;
+                } else {
+                    const float2 q = static_cast<const float2*>(xfall)[jj];
+                    dx = xr[0] - q.x;
+                    dy_ = xr[1] - q.y;
                 }
+                d2f[u] = dx * dx + dy_ * dy_ + dz_ * dz_;
+            }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int jj = j + u;
-                    if (jj >= j1 || jj == self || !(d2f[u] <= hi2)) continue;
-                    bool in = d2f[u] < lo2;
-                    if (!in) {  // rounding band: the FP64 rule decides
-                        double xj[3];
-                        load_x<D>(xall, jj, xj);
-                        double rv[3];
-                        in = dist2_exact<D>(xi, xj, rv) <= R2;
-                    }
-                    if (in) f(jj);
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j + u;
+                if (jj >= j1 || jj == self || !(d2f[u] <= hi2)) continue;
+                bool in = d2f[u] < lo2;
+                if (!in) {  // rounding band: the FP64 rule decides
+                    double xj[3];
+                    load_x<D>(xall, jj, xj);
+                    double rv[3];
+                    in = dist2_exact<D>(xi, xj, rv) <= R2;
                 }
+                if (in) f(jj);
             }
         }
-    }
+    });
 }
 
 // Records the neighbours of particle i in the chunked list `list`; returns
@@ -641,6 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_clear(const SceneDev* __restric
             const int c = tile.begin + q * kBlock + (int)threadIdx.x;
             if (c <= s.ncell) s.count[c] = 0;
         }
+        if (tile.begin == 0 && threadIdx.x == 0) s.dyn->outside = 0;
     }
 }
 
@@ -718,6 +750,16 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
         }
         key = dead ? s.ncell : cell_key(s, D, x);
         if (!finite) atomicOr(&s.dyn->err, 1);
+        {
+            // particles beyond the grid share its border cells (clamped):
+            // counted, since many of them make those cells crowded
+            bool far = false;
+            for (int k = 0; k < D; ++k) {
+                const double f = floor((x[k] - s.lo[k]) * s.inv_cs);
+                far = far || !(f >= -1.0 && f <= (double)s.nc[k]);
+            }
+            if (far && !dead) atomicAdd(&s.dyn->outside, 1);
+        }
         s.key[i] = key;
         s.slot[i] = wave_counted_slot(s.count, key);
         if (spawn_key >= 0) {

@@ -305,6 +305,25 @@ extern "C" int esph_server_advance(esph_server* sv, int64_t max_steps, int64_t* 
         Request& r = *sv->reqs[sv->active[q]];
         r.done += k;
         r.scene->step += k;
+        // a request whose simulation blew up (non-finite positions, or many
+        // particles far outside its grid, which crowd the border cells) fails
+        // now instead of slowing every member of the batch down
+        SceneDyn dyn{};
+        if (hipMemcpy(&dyn, r.scene->h.dyn, sizeof(SceneDyn), hipMemcpyDeviceToHost) == hipSuccess &&
+            ((dyn.err & 1) || dyn.outside > std::max(64, dyn.n_live / 64))) {
+            esph_scene_destroy(r.scene);
+            r.scene = nullptr;
+            r.state = ESPH_REQ_FAILED;
+            r.t_end = now_s();
+            sv->failed++;
+            r.fluid = FluidCopy();
+            r.inlet = FluidCopy();
+            r.outlet = FluidCopy();
+            r.walls.clear();
+            sv->active.erase(sv->active.begin() + q);
+            sv->dirty = true;
+            continue;
+        }
         if (r.done >= r.nsteps) {
             retire(sv, r);
             sv->active.erase(sv->active.begin() + q);

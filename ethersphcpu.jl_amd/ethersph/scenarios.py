@@ -116,6 +116,15 @@ def _walls(kind, X, normal, gap, v=None, t=None, kappa=None):
     return w
 
 
+def viscous_dt(dt, h, nu):
+    """The examples' acoustic step 0.1 R / c0 (or 0.1 h / c0), capped by the
+    explicit viscous limit 0.125 h^2 / nu: scenes rescaled to finer spacing
+    or lower Reynolds number than the examples' own would otherwise take
+    steps the forward-Euler viscous term cannot sustain.  At the examples'
+    own parameters the acoustic step is the smaller one and is kept."""
+    return min(dt, 0.125 * h * h / nu) if nu > 0 else dt
+
+
 def _ipow(x, n):
     y = np.ones_like(x)
     while n > 0:
@@ -271,6 +280,7 @@ def lid_driven_cavity_2d(dx=0.01, reynolds=100.0, lid_velocity=1.0, kernel=Wendl
     sk = SmoothKernel(R, dim, kernel if not isinstance(kernel, str) else KERNELS[kernel])
     model = CommonWeaklyCompressibleLiquidModel(rho_0, c_0, p_0, gamma, mu_0, [0.0, 0.0])
     dt = 0.1 * sk.influence_radius_ / c_0
+    dt = viscous_dt(dt, sk.h_, model.mu_0_ / model.rho_0_)
     td = DensityReinitializedFixedTimeIntervalForwardEuler(dt, total_step, 100, 0, reinit)
     n = round(L / dx)
     f = LiquidParticles(dim, n * n)
@@ -341,6 +351,7 @@ def poiseuille_2d(rows=50, reynolds=None, l=1e-2, nu=1e-6, rho_0=1000.0, f=8e-5,
     gamma = 7
     model = CommonWeaklyCompressibleLiquidModel(rho_0, c_0, p_0, gamma, mu_0, [f, 0.0])
     dt = 0.1 * sk.influence_radius_ / c_0
+    dt = viscous_dt(dt, sk.h_, model.mu_0_ / model.rho_0_)
     td = DensityReinitializedFixedTimeIntervalForwardEuler(dt, total_step, 100, check, reinit)
     x0 = y0 = 0.0
     fluid_width = l * length_factor
@@ -423,6 +434,7 @@ def poiseuille_3d(cells=40, l=1e-2, nu=1e-6, rho_0=1000.0, f=8e-5, kernel=Wendla
     gamma = 7
     model = CommonWeaklyCompressibleLiquidModel(rho_0, c_0, p_0, gamma, mu_0, [f, 0.0, 0.0])
     dt = 0.1 * sk.influence_radius_ / c_0
+    dt = viscous_dt(dt, sk.h_, model.mu_0_ / model.rho_0_)
     td = DensityReinitializedFixedTimeIntervalForwardEuler(dt, total_step, 10, check, reinit)
     x0 = y0 = z0 = 0.0
     pipe_len = length_factor * l

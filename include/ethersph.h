@@ -287,10 +287,14 @@ int esph_scene_probe(esph_scene* scene, int32_t field, int64_t npoints, const do
    Timing is off unless enabled (it adds events to the stream). */
 int esph_scene_enable_timing(esph_scene* scene, int32_t enable);
 int esph_scene_phase_times(esph_scene* scene, double* ms5, int64_t* steps);
-/* out4: [0] particle-steps whose neighbour count exceeded the per-particle
+/* out6: [0] particle-steps whose neighbour count exceeded the per-particle
    list width (those particles scanned their cells instead; results are the
-   same), [1] fluid-fluid list width, [2] fluid-wall list width, [3] cells. */
-int esph_scene_stats(esph_scene* scene, int64_t* out4);
+   same), [1] fluid-fluid list width, [2] fluid-wall list width, [3] cells,
+   [4] particles more than one cell outside the grid at the last sort (they
+   share the border cells; many of them mean a blown-up simulation and slow
+   steps), [5] error bits (1: non-finite position, 2: inlet copies dropped at
+   capacity). */
+int esph_scene_stats(esph_scene* scene, int64_t* out6);
 /* Graph mode: the first step captures one step as a HIP graph; later steps
    replay it (the Shepard pass is then launched every step and skips itself
    on the device when not due).  Phase timing is not recorded in graph mode. */
@@ -319,8 +323,10 @@ int esph_batch_sync(esph_batch* batch);
    (graph-replayed) by min(max_steps, fewest steps left among them), then reads
    back and frees finished requests (their x, v, rho, p, c, id [, t] stay
    available through esph_server_result until released).  A request that
-   can never fit the budget fails.  Not thread-safe: one host thread drives a
-   server. */
+   can never fit the budget fails, and so does a running request whose
+   simulation blows up (a non-finite position, or more than max(64, n/64)
+   particles far outside its grid).  Not thread-safe: one host thread drives
+   a server. */
 enum { ESPH_REQ_QUEUED = 0, ESPH_REQ_RUNNING = 1, ESPH_REQ_DONE = 2, ESPH_REQ_FAILED = 3 };
 typedef struct esph_server esph_server;
 int esph_server_create(esph_ctx* ctx, int32_t max_active, esph_server** out); /* max_active <= 0: no limit */

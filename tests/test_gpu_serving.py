@@ -92,3 +92,22 @@ def test_admission_waits_for_budget():
     assert np.array_equal(sv.result(a, _lib.FIELD_X), sv.result(b, _lib.FIELD_X))
     sv.close()
     ctx.close()
+
+
+def test_blown_up_request_fails_without_stalling_the_batch(ctx):
+    """A request stepped with a time step far beyond the viscous limit blows
+    up; the server fails it (non-finite or far-escaped particles) while a
+    well-posed request in the same batch still completes."""
+    bad = scenarios.poiseuille_2d(rows=12, length_factor=2)
+    bad.td.dt_ *= 40.0
+    good = scenarios.collapse_dry_2d(dx=0.05)
+    sv = Server(ctx)
+    a = sv.submit_scenario(bad, _lib.FP32_BULK, 3000)
+    b = sv.submit_scenario(good, _lib.FP32_BULK, 300)
+    for _ in range(100):
+        if all(sv.status(r)[0] in (_lib.REQ_DONE, _lib.REQ_FAILED) for r in (a, b)):
+            break
+        sv.advance(30)
+    assert sv.status(a)[0] == _lib.REQ_FAILED
+    assert sv.status(b) == (_lib.REQ_DONE, 300)
+    sv.close()
