@@ -66,6 +66,9 @@ def time_gpu(sc, prec, steps, warmup):
     wall = time.perf_counter() - t0
     ph = dev.phase_times()
     ph["stats"] = dev.stats()
+    tuned = dev.ctx.tuned_depth(sc.dim, sc.kernel.kind, prec, 1)
+    if tuned is not None:
+        ph["gather_depth"] = {"chosen": tuned[0], "forces_ms_at_depth_1_2_4": tuned[1]}
     n = dev.fluid_count()
     dev.close()
     return wall, ph, n

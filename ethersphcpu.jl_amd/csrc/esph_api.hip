@@ -103,6 +103,29 @@ extern "C" int esph_ctx_destroy(esph_ctx* ctx) {
 
 extern "C" const char* esph_last_error(const esph_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+extern "C" int esph_ctx_set_tuning(esph_ctx* ctx, int32_t mode, int32_t depth) {
+    if (!ctx || (mode != 0 && mode != 1) || (mode == 0 && depth != 1 && depth != 2 && depth != 4))
+        return fail(ctx, ESPH_ERR_ARG, "set_tuning: mode 0 needs depth 1, 2 or 4; mode 1 tunes");
+    ctx->tune = mode;
+    ctx->fixed_depth = mode == 0 ? depth : 2;
+    ctx->depth.clear();
+    ctx->depth_ms.clear();
+    return ESPH_OK;
+}
+
+extern "C" int esph_ctx_tuned_depth(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision,
+                                    int32_t thermal_mix, int32_t* depth, float* ms3) {
+    if (!ctx || !depth) return ESPH_ERR_ARG;
+    auto it = ctx->depth.find({dim, kernel, precision, thermal_mix});
+    if (it == ctx->depth.end()) return ESPH_ERR_STATE;
+    *depth = it->second;
+    if (ms3) {
+        const auto& m = ctx->depth_ms.at(it->first);
+        for (int q = 0; q < 3; ++q) ms3[q] = m[q];
+    }
+    return ESPH_OK;
+}
+
 extern "C" int esph_ctx_memory_used(const esph_ctx* ctx, uint64_t* bytes) {
     if (!ctx || !bytes) return ESPH_ERR_ARG;
     *bytes = ctx->used;
@@ -696,7 +719,8 @@ extern "C" int esph_scene_step(esph_scene* s, int64_t nsteps) {
         s->step += nsteps;
         return ESPH_OK;
     }
-    const Bucket b = scene_bucket(s);
+    Bucket b = scene_bucket(s);
+    apply_depth(ctx, b);
     const SortPlan plan{s->dev, 1, s->ptiles, s->nptiles, s->ctiles, s->nctiles};
     const int64_t K = s->d.density_reinitialized_interval;
     for (int64_t t = 0; t < nsteps; ++t) {
@@ -707,6 +731,7 @@ extern "C" int esph_scene_step(esph_scene* s, int64_t nsteps) {
             (void)hipEventRecord(ev[0], st);
         }
         hipError_t e = launch_continuity(b, st);
+        if (e == hipSuccess && b.depth == 0) e = tune_depth(ctx, b, st);  // first step of an untuned class
         if (s->timing) (void)hipEventRecord(ev[1], st);
         if (e == hipSuccess) e = launch_forces(b, st);
         if (s->timing) (void)hipEventRecord(ev[2], st);

@@ -23,6 +23,55 @@ using namespace esph;
 
 namespace esph {
 
+std::array<int, 4> depth_key(const Bucket& b) {
+    return {b.dim, b.kernel, b.prec, (b.has_thermal ? 2 : 0) | (b.has_plain ? 1 : 0)};
+}
+
+void apply_depth(const esph_ctx* ctx, Bucket& b) {
+    if (!ctx->tune) {
+        b.depth = ctx->fixed_depth;
+        return;
+    }
+    auto it = ctx->depth.find(depth_key(b));
+    b.depth = it == ctx->depth.end() ? 0 : it->second;
+}
+
+hipError_t tune_depth(esph_ctx* ctx, Bucket& b, hipStream_t st) {
+    const int cand[3] = {1, 2, 4};
+    std::array<float, 3> best{};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int c = 0; c < 3 && e == hipSuccess; ++c) {
+        b.depth = cand[c];
+        e = launch_forces(b, st);  // warm-up
+        float t = 1e30f;
+        for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+            e = hipEventRecord(e0, st);
+            if (e == hipSuccess) e = launch_forces(b, st);
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            t = std::min(t, ms);
+        }
+        best[c] = t;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) {
+        b.depth = 2;
+        return e;
+    }
+    int k = 0;
+    for (int c = 1; c < 3; ++c)
+        if (best[c] < best[k]) k = c;
+    b.depth = cand[k];
+    ctx->depth[depth_key(b)] = b.depth;
+    ctx->depth_ms[depth_key(b)] = best;
+    return hipSuccess;
+}
+
 int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
     release();
     const int n = (int)members.size();
@@ -75,9 +124,11 @@ int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
         return hip_fail(ctx, "batch upload", e);
     }
     buckets.clear();
-    for (const Off& o : offs)
+    for (const Off& o : offs) {
         buckets.push_back(Bucket{o.dim, o.kernel, o.prec, dtable, dtiles + o.p, (int)o.np, dtiles + o.o, o.ns, o.th,
                                  o.pl});
+        apply_depth(ctx, buckets.back());
+    }
     plan = SortPlan{dtable, n, dtiles, (int)all_p.size(), dtiles + all_p.size(), (int)all_c.size()};
     nmembers = n;
     return ESPH_OK;
@@ -97,7 +148,41 @@ hipError_t StepPlan::launch_step(hipStream_t st) const {
 
 int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
     hipStream_t st = ctx->stream;
+    // buckets of an untuned class: the first step runs eagerly and tunes
+    // their gather depth between its continuity and force passes
+    bool untuned = false;
+    for (Bucket& b : buckets) {
+        apply_depth(ctx, b);
+        untuned = untuned || b.depth == 0;
+    }
+    if (untuned && nsteps > 0) {
+        hipError_t e = hipSuccess;
+        for (const Bucket& b : buckets)
+            if (e == hipSuccess) e = launch_continuity(b, st);
+        for (Bucket& b : buckets)
+            if (e == hipSuccess && b.depth == 0) {
+                e = tune_depth(ctx, b, st);
+                for (Bucket& o : buckets) apply_depth(ctx, o);  // same class, same depth
+            }
+        for (const Bucket& b : buckets)
+            if (e == hipSuccess) e = launch_forces(b, st);
+        for (const Bucket& b : buckets)
+            if (e == hipSuccess) e = launch_shepard(b, st);
+        if (e == hipSuccess) e = launch_sort(plan, 0, st);
+        if (e != hipSuccess) return hip_fail(ctx, "tuning step", e);
+        nsteps -= 1;
+    }
+    // a graph captured with other depths (tuning settings changed) is recaptured
+    std::vector<int> depths;
+    for (const Bucket& b : buckets) depths.push_back(b.depth);
+    if (exec && depths != captured_depth) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
     if (use_graph && !exec && nsteps > 0) {
+        captured_depth = depths;
         hipGraph_t g = nullptr;
         HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = launch_step(st);

@@ -1,0 +1,423 @@
+// esph_common.hpp — device helpers shared by the gfx950 kernels of the
+// per-step SPH hot path: XCD-aware tile order, cell-grid candidate rows,
+// neighbour lists, packed pair records and launch dispatch.
+#pragma once
+
+#include "esph_device.hpp"
+#include "esph_launch.hpp"
+
+#include <type_traits>
+
+namespace esph {
+
+// Workgroups go to the eight XCDs round-robin (block b runs on XCD b % 8).
+// With one tile per block, xcd_tile gives each XCD one contiguous range of
+// tiles, walked in order, so each XCD's L2 holds one compact, slowly moving
+// region of the cell-sorted particles instead of every eighth tile.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    constexpr int X = 8;
+    const int per = n / X, rem = n % X;
+    const int x = b % X, k = b / X;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
+}
+__device__ __forceinline__ int tile_index(int b, int n) {
+    return (int)gridDim.x == n ? xcd_tile(b, n) : b;
+}
+
+template <typename T>
+__device__ __forceinline__ T ld(const void* p, int i) {
+    return static_cast<const T*>(p)[i];
+}
+
+template <int D>
+__device__ __forceinline__ void load_x(const double* __restrict__ x, int j, double* o) {
+    if (D == 2) {
+        double2 q = reinterpret_cast<const double2*>(x)[j];
+        o[0] = q.x;
+        o[1] = q.y;
+    } else {
+        o[0] = x[3 * j];
+        o[1] = x[3 * j + 1];
+        o[2] = x[3 * j + 2];
+    }
+}
+
+// Candidate rows of a point xi in cell c: the grid's cells are at least R /
+// stencil wide, so neighbours lie within `stencil` cells on every axis.  For
+// each (dy, dz) the cells of the row are x-contiguous in memory, so a row is
+// one index range [j0, j1).  Rows and row ends whose nearest point is
+// farther than R from xi (judged from xi's offset inside its cell, with a
+// slack that keeps the test conservative under rounding and clamping) are
+// skipped.  Calls row(j0, j1) for each remaining row.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_row(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                             const double* xi, const int c[3], F&& row) {
+    const int m = s.stencil;
+    const double cs = s.cs;
+    const double slack = 1e-7 * cs;
+    const double lim = s.R2 * (1.0 + 1e-6);
+    double p[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p[k] = k < D ? xi[k] - (s.lo[k] + (double)c[k] * cs) : 0.0;
+    // distance from xi to the slab of the cell at offset o along one axis
+    auto dmin = [&](int o, double pk) {
+        double d = o == 0 ? 0.0 : (o > 0 ? (double)(o - 1) * cs + (cs - pk) : (double)(-o - 1) * cs + pk);
+        d -= slack;
+        return d > 0.0 ? d : 0.0;
+    };
+    const int mz = D == 3 ? m : 0;
+    for (int dz = -mz; dz <= mz; ++dz) {
+        const int cz = c[2] + dz;
+        if (cz < 0 || cz >= s.nc[2]) continue;
+        const double rz = D == 3 ? dmin(dz, p[2]) : 0.0;
+        for (int dy = -m; dy <= m; ++dy) {
+            const int cy = c[1] + dy;
+            if (cy < 0 || cy >= s.nc[1]) continue;
+            const double ry = dmin(dy, p[1]);
+            const double r2 = ry * ry + rz * rz;
+            if (r2 > lim) continue;
+            int xl = -m, xh = m;
+            while (xl < 0) {
+                const double d = dmin(xl, p[0]);
+                if (d * d + r2 <= lim) break;
+                ++xl;
+            }
+            while (xh > 0) {
+                const double d = dmin(xh, p[0]);
+                if (d * d + r2 <= lim) break;
+                --xh;
+            }
+            const int cx0 = c[0] + xl > 0 ? c[0] + xl : 0;
+            const int cx1 = c[0] + xh < s.nc[0] - 1 ? c[0] + xh : s.nc[0] - 1;
+            if (cx0 > cx1) continue;
+            row(cstart[cell_linear(s, cx0, cy, cz)], cstart[cell_linear(s, cx1, cy, cz) + 1]);
+        }
+    }
+}
+
+// Iterates the particles of the candidate rows of xi whose distance
+// satisfies the neighbour rule (FP64, exact).  The candidate loop issues the
+// loads of four candidates before testing them.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_neighbour(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                                   const double* __restrict__ xall, const double* xi,
+                                                   const int c[3], int self, F&& f) {
+    const double R2 = s.R2;
+    for_each_row<D>(s, cstart, xi, c, [&](int j0, int j1) {
+        for (int j = j0; j < j1; j += 4) {
+            double xj[4][3];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) load_x<D>(xall, j + u < j1 ? j + u : j1 - 1, xj[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j + u;
+                double rv[3];
+                const double d2 = dist2_exact<D>(xi, xj[u], rv);
+                if (jj < j1 && jj != self && d2 <= R2) f(jj, d2, rv);
+            }
+        }
+    });
+}
+
+// Neighbour lists are stored in chunks of four: entries 4q..4q+3 of particle
+// i are the int4 at [q * cap + i], so a wavefront reads 64 consecutive int4
+// (1 KiB) per chunk row and each lane writes whole 16-byte pieces.
+constexpr int kChunk = 4;
+__device__ __forceinline__ size_t list_at(size_t cap, int k, int i) {
+    return (((size_t)(k >> 2) * cap + (size_t)i) << 2) + (size_t)(k & 3);
+}
+
+// Candidates of particle i over the adjacent cells, tested first in float on
+// the grid-relative positions xf: a candidate whose float distance is clearly
+// inside or clearly outside R is decided there; only those within the
+// rounding band tau of R take the FP64 rule of for_each_neighbour.  tau bounds
+// the float error: 2^-21 (|xr_i|_inf + 2 R), so the decisions equal the FP64
+// rule's.  Calls f(j) for every neighbour, in cell order.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_candidate_f32(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                                       const void* __restrict__ xfall,
+                                                       const double* __restrict__ xall, const double* xi,
+                                                       const int c[3], int self, F&& f) {
+    float xr[3] = {(float)(xi[0] - s.lo[0]), (float)(xi[1] - s.lo[1]), D == 3 ? (float)(xi[2] - s.lo[2]) : 0.0f};
+    const double amax = fmax(fabs((double)xr[0]), fmax(fabs((double)xr[1]), fabs((double)xr[2])));
+    const double tau = ldexp(amax + 2.0 * s.R, -21);
+    const double rin = s.R - tau, rout = s.R + tau;
+    const float lo2 = rin > 0.0 ? (float)(rin * rin) * (1.0f - 0x1p-22f) : -1.0f;
+    const float hi2 = (float)(rout * rout) * (1.0f + 0x1p-22f);
+    const double R2 = s.R2;
+    for_each_row<D>(s, cstart, xi, c, [&](int j0, int j1) {
+        for (int j = j0; j < j1; j += 4) {
+            float d2f[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j + u < j1 ? j + u : j1 - 1;
+                float dx, dy_, dz_ = 0.0f;
+                if (D == 3) {
+                    const float4 q = static_cast<const float4*>(xfall)[jj];
+                    dx = xr[0] - q.x;
+                    dy_ = xr[1] - q.y;
+                    dz_ = xr[2] - q.z;
+                } else {
+                    const float2 q = static_cast<const float2*>(xfall)[jj];
+                    dx = xr[0] - q.x;
+                    dy_ = xr[1] - q.y;
+                }
+                d2f[u] = dx * dx + dy_ * dy_ + dz_ * dz_;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j + u;
+                if (jj >= j1 || jj == self || !(d2f[u] <= hi2)) continue;
+                bool in = d2f[u] < lo2;
+                if (!in) {  // rounding band: the FP64 rule decides
+                    double xj[3];
+                    load_x<D>(xall, jj, xj);
+                    double rv[3];
+                    in = dist2_exact<D>(xi, xj, rv) <= R2;
+                }
+                if (in) f(jj);
+            }
+        }
+    });
+}
+
+// Records the neighbours of particle i in the chunked list `list`; returns
+// the count.  Entries fill a 4-entry register buffer flushed as one int4.
+// Entries beyond `width` (a multiple of 4) are dropped and the particle
+// later scans its cells.
+template <int D>
+__device__ __forceinline__ int build_list(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                          const void* __restrict__ xfall, const double* __restrict__ xall,
+                                          const double* xi, const int c[3], int i, int self,
+                                          int32_t* __restrict__ list, int width) {
+    int cnt = 0;
+    int4 buf = make_int4(0, 0, 0, 0);
+    const size_t cap = (size_t)s.cap;
+    for_each_candidate_f32<D>(s, cstart, xfall, xall, xi, c, self, [&](int j) {
+        const int r = cnt & 3;
+        buf.x = r == 0 ? j : buf.x;
+        buf.y = r == 1 ? j : buf.y;
+        buf.z = r == 2 ? j : buf.z;
+        buf.w = r == 3 ? j : buf.w;
+        if (r == 3 && cnt < width) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
+        ++cnt;
+    });
+    if ((cnt & 3) != 0 && cnt < width) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
+    return cnt;
+}
+
+// Iterates the recorded neighbours of particle i with the same (j, d2, rv)
+// arguments as for_each_neighbour; the next entry's index and position are
+// loaded while the current one is processed.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_listed(const SceneDev& s, const double* __restrict__ xall,
+                                                const double* xi, int i, const int32_t* __restrict__ list,
+                                                int cnt, F&& f) {
+    if (cnt <= 0) return;
+    const size_t cap = (size_t)s.cap;
+    int4 ch = *reinterpret_cast<const int4*>(list + list_at(cap, 0, i));
+    int jn = ch.x;
+    double xn[3];
+    load_x<D>(xall, jn, xn);
+    for (int k = 0; k < cnt; ++k) {
+        const int j = jn;
+        double xj[3] = {xn[0], xn[1], D == 3 ? xn[2] : 0.0};
+        if (k + 1 < cnt) {
+            const int r = (k + 1) & 3;
+            if (r == 0) ch = *reinterpret_cast<const int4*>(list + list_at(cap, k + 1, i));
+            jn = r == 0 ? ch.x : r == 1 ? ch.y : r == 2 ? ch.z : ch.w;
+            load_x<D>(xall, jn, xn);
+        }
+        double rv[3];
+        const double d2 = dist2_exact<D>(xi, xj, rv);
+        f(j, d2, rv);
+    }
+}
+
+// Pairs of particle i: from the step's list, or (SCAN, for particles whose
+// list overflowed) by scanning the cells.  Both visit the identical pair set.
+template <int D, bool SCAN, typename F>
+__device__ __forceinline__ void for_pairs(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                          const double* __restrict__ xall, const double* xi, int i, int self,
+                                          const int32_t* __restrict__ list, int cnt, F&& f) {
+    if (SCAN) {
+        int c[3];
+        cell_of<D>(s, xi, c);
+        for_each_neighbour<D>(s, cstart, xall, xi, c, self, f);
+    } else {
+        for_each_listed<D>(s, xall, xi, i, list, cnt, f);
+    }
+}
+
+// Pipelined form of for_pairs: neighbours are taken DEPTH at a time; the
+// positions and the records load(j) of a group are all issued before any of
+// their arithmetic, so each lane keeps DEPTH gathers in flight.  The list
+// chunk after the current one is fetched while the current one is used.
+// Every slot of a written chunk holds a valid particle index (slots past the
+// count keep earlier entries or zero), so a group may load past the count;
+// body runs only for the counted entries, in list order, so the result does
+// not depend on DEPTH.
+template <int D, bool SCAN, int DEPTH, typename Load, typename Body>
+__device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __restrict__ cstart,
+                                            const double* __restrict__ xall, const double* xi, int i, int self,
+                                            const int32_t* __restrict__ list, int cnt, Load&& load, Body&& body) {
+    static_assert(DEPTH == 1 || DEPTH == 2 || DEPTH == 4, "DEPTH divides a 4-entry chunk");
+    if (SCAN) {
+        int c[3];
+        cell_of<D>(s, xi, c);
+        for_each_neighbour<D>(s, cstart, xall, xi, c, self,
+                              [&](int j, double d2, const double* rv) { body(j, d2, rv, load(j)); });
+        return;
+    }
+    if (cnt <= 0) return;
+    const size_t cap = (size_t)s.cap;
+    const int nq = (cnt + 3) >> 2;
+    int4 cur = *reinterpret_cast<const int4*>(list + list_at(cap, 0, i));
+    for (int q = 0; q < nq; ++q) {
+        int4 nxt = cur;
+        if (q + 1 < nq) nxt = *reinterpret_cast<const int4*>(list + list_at(cap, 4 * (q + 1), i));
+        const int js[4] = {cur.x, cur.y, cur.z, cur.w};
+        const int m = cnt - 4 * q;  // entries of this chunk: min(4, m)
+#pragma unroll
+        for (int g = 0; g < 4; g += DEPTH) {
+            if (g < m) {
+                double xj[DEPTH][3];
+                decltype(load(0)) rec[DEPTH];
+#pragma unroll
+                for (int u = 0; u < DEPTH; ++u) {
+                    load_x<D>(xall, js[g + u], xj[u]);
+                    rec[u] = load(js[g + u]);
+                }
+#pragma unroll
+                for (int u = 0; u < DEPTH; ++u) {
+                    if (g + u < m) {
+                        double rv[3];
+                        const double d2 = dist2_exact<D>(xi, xj[u], rv);
+                        body(js[g + u], d2, rv, rec[u]);
+                    }
+                }
+            }
+        }
+        cur = nxt;
+    }
+}
+
+// The SCAN instances walk the step's overflow list with a small fixed grid:
+// kOvfTiles tiles of kBlock entries per scene, striding over the list.
+// Calls f(i) for the particle(s) of this thread: the tile slot (main
+// kernels) or, in the SCAN instances, every kOvfTiles*kBlock-th entry of the
+// step's overflow list.
+template <bool SCAN, typename F>
+__device__ __forceinline__ void for_work(const SceneDev& s, const SceneDyn& dyn, const Tile& tile, F&& f) {
+    const int q0 = tile.begin + (int)threadIdx.x;
+    if (SCAN) {
+        for (int q = q0; q < dyn.novf; q += kOvfTiles * kBlock) f(s.ovf[q]);
+    } else if (q0 < dyn.n_live) {
+        f(q0);
+    }
+}
+
+__device__ __forceinline__ bool overflowed(const SceneDev& s, int i) {
+    return s.nnb[i] > s.kff || (s.nw > 0 && s.nnbw[i] > s.kfw);
+}
+
+// Velocity record of a neighbour (continuity pass).
+template <typename T>
+struct ARec {
+    T v[3];
+    T m, gap;
+};
+
+// Unpacked neighbour record (see FluidSet).
+template <typename T>
+struct Rec {
+    T v[3];
+    T m, rho, pr2, gap, Tt, kap;
+};
+
+template <int D, typename T>
+__device__ __forceinline__ void load_A(const void* A, int j, T* v, T& m, T& gap) {
+    using V = typename Vec4<T>::type;
+    const V a = static_cast<const V*>(A)[j];
+    v[0] = a.x;
+    v[1] = a.y;
+    if (D == 3) {
+        v[2] = a.z;
+        m = a.w;
+    } else {
+        m = a.z;
+        gap = a.w;
+    }
+}
+
+template <int D, typename T, bool THERMAL>
+__device__ __forceinline__ void load_rec(const FluidSet& S, int j, Rec<T>& r) {
+    using V = typename Vec4<T>::type;
+    load_A<D, T>(S.A, j, r.v, r.m, r.gap);
+    const V b = static_cast<const V*>(S.B)[j];
+    r.rho = b.x;
+    r.pr2 = b.y;
+    if (D == 3) {
+        r.gap = b.z;
+        r.Tt = b.w;
+        if (THERMAL) r.kap = ld<T>(S.kap, j);
+    } else {
+        r.Tt = b.z;
+        r.kap = b.w;
+    }
+}
+
+// Record fields written by the sort (A, B.zw, kap) for particle p.
+template <typename T>
+__device__ __forceinline__ void store_sorted_rec(const FluidSet& O, int D, int p, const double* v, double m,
+                                                 double gap, double Tt, double kap, bool thermal) {
+    T* A = static_cast<T*>(O.A) + 4 * (size_t)p;
+    T* B = static_cast<T*>(O.B) + 4 * (size_t)p;
+    if (D == 3) {
+        A[0] = (T)v[0]; A[1] = (T)v[1]; A[2] = (T)v[2]; A[3] = (T)m;
+        B[2] = (T)gap;
+        B[3] = (T)Tt;
+        if (thermal) static_cast<T*>(O.kap)[p] = (T)kap;
+    } else {
+        A[0] = (T)v[0]; A[1] = (T)v[1]; A[2] = (T)m; A[3] = (T)gap;
+        B[2] = (T)Tt;
+        B[3] = (T)kap;
+    }
+}
+
+__device__ __forceinline__ bool check_due(const SceneDev& s, const SceneDyn& d) {
+    return (s.flags & FLAG_CHECK) && s.kcheck > 0 && (d.step + 1) % s.kcheck == 0;
+}
+__device__ __forceinline__ bool shepard_due(const SceneDev& s, const SceneDyn& d) {
+    return s.kreinit > 0 && (d.step + 1) % s.kreinit == 0;
+}
+
+#define ESPH_DISPATCH_K(KVAR, BODY)                                                 \
+    switch (KVAR) {                                                                 \
+        case ESPH_CUBIC_SPLINE: { constexpr int K = ESPH_CUBIC_SPLINE; BODY; } break; \
+        case ESPH_GAUSSIAN: { constexpr int K = ESPH_GAUSSIAN; BODY; } break;         \
+        case ESPH_WENDLAND_C2: { constexpr int K = ESPH_WENDLAND_C2; BODY; } break;   \
+        case ESPH_WENDLAND_C4: { constexpr int K = ESPH_WENDLAND_C4; BODY; } break;   \
+        default: return hipErrorInvalidValue;                                       \
+    }
+
+// Gather depth of the pair passes (neighbours whose loads are in flight
+// together): 1, 2 or 4; chosen per bucket by the tuner.
+#define ESPH_DISPATCH_DEPTH(DVAR, BODY)                     \
+    switch (DVAR) {                                         \
+        case 1: { constexpr int DP = 1; BODY; } break;      \
+        case 4: { constexpr int DP = 4; BODY; } break;      \
+        default: { constexpr int DP = 2; BODY; } break;     \
+    }
+
+static inline int grid_for(int ntiles) { return ntiles > 0 ? ntiles : 1; }
+
+template <typename L>
+static hipError_t launch_both(const Bucket& b, hipStream_t st, L&& launch) {
+    if (b.ntiles <= 0) return hipSuccess;
+    launch(std::false_type{}, dim3(grid_for(b.ntiles)), b.tiles, b.ntiles);
+    launch(std::true_type{}, dim3(kOvfTiles * b.nscenes), b.ovf_tiles, kOvfTiles * b.nscenes);
+    return hipGetLastError();
+}
+
+}  // namespace esph

@@ -20,6 +20,7 @@ struct Bucket {
     int nscenes;
     bool has_thermal;        // some scene of the bucket is thermal
     bool has_plain;          // some scene of the bucket is not
+    int depth = 2;           // gather depth of the pair passes (1, 2 or 4; see the tuner)
 };
 
 // Tables for the dimension-agnostic sort kernels of a whole batch.
@@ -32,6 +33,12 @@ struct SortPlan {
     int nctiles;
 };
 
+template <int D>
+hipError_t launch_continuity_d(const Bucket& b, hipStream_t st);
+template <int D>
+hipError_t launch_forces_d(const Bucket& b, hipStream_t st);
+template <int D>
+hipError_t launch_shepard_d(const Bucket& b, hipStream_t st);
 hipError_t launch_continuity(const Bucket& b, hipStream_t st);
 hipError_t launch_forces(const Bucket& b, hipStream_t st);
 hipError_t launch_shepard(const Bucket& b, hipStream_t st);

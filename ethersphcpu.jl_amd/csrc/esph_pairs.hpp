@@ -1,0 +1,416 @@
+// esph_pairs.hpp — pair kernels of one step (continuity, forces +
+// integration, Shepard) and their launchers, templated on the dimension;
+// esph_pairs2.hip and esph_pairs3.hip instantiate them for 2D and 3D.
+#pragma once
+
+#include "esph_common.hpp"
+
+namespace esph {
+// ---------------------------------------------------------------------------
+// continuity! (Interaction/Continuity.jl:8-17), pull form:
+//   drho_i = sum_j m_j (v_i - v_j) . gradW_ij,  gradW_ij = W'(r) r_ij / r
+// then updateDensity! (SelfAction/UpdateDensity.jl:8-12) and updatePressure!
+// (SelfAction/UpdatePressure.jl:8-16).  Writes rho, p, c and the record's
+// (rho, p/rho^2).  This pass also records the step's neighbour lists
+// (positions of the step start), which the force and Shepard passes reuse.
+// ---------------------------------------------------------------------------
+template <int D, int K, typename T, bool SCAN, int DEPTH>
+__global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restrict__ scenes,
+                                                       const Tile* __restrict__ tiles, int ntiles) {
+    using KQ = KernelQ<T, K>;
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
+        const Tile tile = tiles[t];
+        const SceneDev& s = scenes[tile.scene];
+        const SceneDyn dyn = *s.dyn;
+        for_work<SCAN>(s, dyn, tile, [&](int i) {
+        const FluidSet& S = s.set[dyn.parity];
+        if (S.kind && S.kind[i] != KIND_FLUID) {
+            // inlet/outlet buffers keep rho and p (drho is reset,
+            // FluidInletOutletWallSolve.jl:91-99) and need no lists
+            s.nnb[i] = 0;
+            if (s.nw > 0) s.nnbw[i] = 0;
+            return;
+        }
+        const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
+        double xi[3];
+        load_x<D>(S.x, i, xi);
+        T vi[3], m_i, gap_i;
+        load_A<D, T>(S.A, i, vi, m_i, gap_i);
+        int cnt = 0;
+        if (!SCAN) {
+            int c[3];
+            cell_of<D>(s, xi, c);
+            cnt = build_list<D>(s, s.cell_start, S.xf, S.x, xi, c, i, i, s.nbr, s.kff);
+            s.nnb[i] = cnt;
+            bool over = cnt > s.kff;
+            if (s.nw > 0) {
+                const int cw = build_list<D>(s, s.wcell_start, s.wxf, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
+                s.nnbw[i] = cw;
+                over = over || cw > s.kfw;
+            }
+            if (over) {
+                // finished by the SCAN instance of this kernel
+                s.ovf[atomicAdd(&s.dyn->novf, 1)] = i;
+                atomicAdd(&s.dyn->overflow, 1);
+                return;
+            }
+        }
+        T drho = T(0);
+        auto load = [&](int j) {
+            ARec<T> a;
+            load_A<D, T>(S.A, j, a.v, a.m, a.gap);
+            return a;
+        };
+        for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.nbr, cnt, load,
+                                    [&](int, double d2, const double* rv, const ARec<T>& a) {
+            const T* vj = a.v;
+            const T m_j = a.m;
+            const T r = psqrt((T)d2);
+            T W, dW;
+            KQ::wdw(r * inv_h, kA, kB, W, dW);
+            // normalize(0) is NaN and the reference zeroes it (Neighbour.jl:54-59)
+            const T sg = d2 == 0.0 ? T(0) : dW * prcp(r);
+            T dot = T(0);
+#pragma unroll
+            for (int k = 0; k < D; ++k) dot += (vi[k] - vj[k]) * (T)rv[k];
+            drho += m_j * sg * dot;
+        });
+        {
+#pragma clang fp contract(off)
+            double rho = S.rho[i];
+            rho += (double)drho * s.dt;
+            const double rr = rho / s.rho0;
+            const double p = s.b * (ipow(rr, s.gamma) - 1.0) + s.p0;
+            const double cc = s.c0 * sqrt(ipow(rr, s.gamma - 1));
+            S.rho[i] = rho;
+            S.p[i] = p;
+            S.c[i] = cc;
+            T* B = static_cast<T*>(S.B) + 4 * (size_t)i;
+            B[0] = (T)rho;
+            B[1] = (T)(p / (rho * rho));
+        }
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Momentum (and heat) pass + integration, pull form over the step's lists.
+//  FF: pressureForce! (PressureForce.jl:8-21), viscosityForce!
+//      (ViscosityForce.jl:8-25), thermalConduction! (ThermalConduction.jl:8-22)
+//  FW: wallForce! (WallForce.jl:8-30), pressureForce! wall
+//      (PressureForce.jl:23-35, solvers other than FluidWallSolve),
+//      viscosityForce! wall (ViscosityForce.jl:27-46), thermalConduction!
+//      thermostat (ThermalConduction.jl:24-36)
+//  then updateTemperature!, updateVelocity!(…, body force), updatePosition!
+//  (UpdateTemperature.jl:8-12, UpdateVelocity.jl:8-23, UpdatePosition.jl:8-14).
+// The pair terms are the reference's, regrouped so that each pair needs at
+// most two reciprocals:
+//   viscosity  4 (mu_i + mu_j) r W' / (rho_i + rho_j)^2 / (r^2 + 0.01 h^2)
+//              = 4 nu0 r W' / ((rho_i + rho_j) (r^2 + 0.01 h^2))
+//   conduction 2 k_eq r W' (T_i - T_j) / rho_i / rho_j / (r^2 + .01 h^2) m_j / cp_i
+//              = 4 k_i k_j r W' (T_i - T_j) m_j / ((k_i + k_j) rho_j (r^2 + .01 h^2)) / (rho_i cp_i)
+// and 1 / W(gap) is taken per particle when both gaps are equal.
+// New x, v, T go to xn, vn, Tn in the current particle order.
+// ---------------------------------------------------------------------------
+// Buffer particles of the inlet/outlet solver: dv = 0, then updatePosition!
+// (FluidInletOutletWallSolve.jl:170-186): x += v dt - 0 dt^2 / 2.
+template <int D>
+__device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S, int i) {
+#pragma clang fp contract(off)
+    const double dt = s.dt;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double v = S.v[i * D + k];
+        s.vn[i * D + k] = v;
+        s.xn[i * D + k] = S.x[i * D + k] + (v * dt - 0.0 * (dt * dt) / 2.0);
+    }
+}
+
+template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH>
+__device__ __forceinline__ void forces_body(const SceneDev& s, const FluidSet& S, int i) {
+    using KQ = KernelQ<T, K>;
+    if (S.kind && S.kind[i] != KIND_FLUID) {
+        buffer_move<D>(s, S, i);
+        return;
+    }
+    const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
+    const T eta2 = (T)(0.01 * (s.h * s.h));
+    const T nu0 = (T)s.nu0;
+    double xi[3];
+    load_x<D>(S.x, i, xi);
+    Rec<T> me;
+    load_rec<D, T, THERMAL>(S, i, me);
+    const T rho_i = me.rho, pr2_i = me.pr2, gap_i = me.gap, m_i = me.m, T_i = me.Tt, kap_i = me.kap;
+    T W_gap, dW_gap;
+    KQ::wdw(gap_i * inv_h, kA, kB, W_gap, dW_gap);
+    const T inv_Wgap_i = prcp(W_gap);
+    T dv[3] = {T(0), T(0), T(0)};
+    T dT = T(0);
+    auto load = [&](int j) {
+        Rec<T> o;
+        load_rec<D, T, THERMAL>(S, j, o);
+        return o;
+    };
+    for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.nbr, s.nnb[i], load,
+                                [&](int, double d2, const double* rv, const Rec<T>& o) {
+        const T r = psqrt((T)d2);
+        T W, dW;
+        KQ::wdw(r * inv_h, kA, kB, W, dW);
+        const T sg = d2 == 0.0 ? T(0) : dW * prcp(r);
+        // pressure, with the 1% correction W(r) / W((gap_i + gap_j) / 2)
+        T P = pr2_i + o.pr2;
+        T inv_Wg = inv_Wgap_i;
+        if (o.gap != gap_i) {
+            T w2, d2w;
+            KQ::wdw((gap_i + o.gap) * T(0.5) * inv_h, kA, kB, w2, d2w);
+            inv_Wg = prcp(w2);
+        }
+        P += fabs(P) * T(0.01) * W * inv_Wg;
+        // viscosity
+        const T r2e = r * r + eta2;
+        const T vf = T(4) * nu0 * r * dW * prcp((rho_i + o.rho) * r2e);
+        const T ps = -P * sg;
+#pragma unroll
+        for (int k = 0; k < D; ++k) dv[k] += o.m * (ps * (T)rv[k] + vf * (me.v[k] - o.v[k]));
+        if (THERMAL)
+            dT += T(4) * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e);
+    });
+    if (THERMAL) dT *= prcp(rho_i * (T)S.cp[i]);
+    if (s.nw > 0) {
+        using V = typename Vec4<T>::type;
+        const bool wall_pressure = (s.flags & FLAG_WALL_PRESSURE) != 0;
+        const T c_i = (T)S.c[i];
+        const T P0 = (T)(s.p0 / (s.rho0 * s.rho0));
+        const T inv_rho_i = prcp(rho_i);
+        const T th_w = (T)(1.0 / (s.cp * s.rho0)) * inv_rho_i * m_i;
+        T dTw = T(0);
+        struct WRec {
+            V wa, wb;
+            T kw;
+        };
+        auto wload = [&](int j) {
+            WRec w;
+            w.wa = static_cast<const V*>(s.wA)[j];
+            w.wb = static_cast<const V*>(s.wB)[j];
+            w.kw = (D == 3 && THERMAL) ? ld<T>(s.wkap, j) : T(0);
+            return w;
+        };
+        for_pairs_d<D, SCAN, DEPTH>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], wload,
+                                    [&](int j, double d2, const double* rv, const WRec& w) {
+            const V wa = w.wa;
+            const V wb = w.wb;
+            T n[3], vw[3], gap_j, Tw, kw;
+            n[0] = wa.x;
+            n[1] = wa.y;
+            vw[0] = wb.x;
+            vw[1] = wb.y;
+            if (D == 3) {
+                n[2] = wa.z;
+                gap_j = wa.w;
+                vw[2] = wb.z;
+                Tw = wb.w;
+                kw = w.kw;
+            } else {
+                gap_j = wa.z;
+                Tw = wa.w;
+                kw = wb.z;
+            }
+            const T r = psqrt((T)d2);
+            T W, dW;
+            KQ::wdw(r * inv_h, kA, kB, W, dW);
+            const T sg = d2 == 0.0 ? T(0) : dW * prcp(r);
+            T vv[3];
+            T vn = T(0), rn = T(0);
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                vv[k] = me.v[k] - vw[k];
+                vn += vv[k] * n[k];
+                rn += (T)rv[k] * n[k];
+            }
+            // wallForce! (Rogers & Dalrymple 2008)
+            {
+                const T psi = fabs(rn);
+                const T a = r * r - psi * psi;
+                const T xi_ = psqrt(a > T(0) ? a : T(0));
+                const T q = psi / gap_j;
+                if (!(q > T(1) || xi_ > gap_j)) {
+                    const T p_xi = fabs((T(1) + cos(T(3.14159265358979323846) * xi_ / gap_j)) * T(0.5));
+                    const T beta = vn > T(0) ? T(0) : T(1);
+                    const T wf = (T(0.01) * (c_i * c_i) + beta * c_i * fabs(vn)) * inv_h * fabs(T(1) - q) *
+                                 prsqrt(q) * p_xi;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) dv[k] += wf * n[k];
+                }
+            }
+            const T r2e = r * r + eta2;
+            const T vf = T(2) * nu0 * r * dW * inv_rho_i * prcp(r2e);
+            T ps = T(0);
+            if (wall_pressure) {
+                T P = pr2_i + P0;
+                P += fabs(P) * T(0.01) * W * inv_Wgap_i;
+                ps = -P * sg;
+            }
+#pragma unroll
+            for (int k = 0; k < D; ++k) dv[k] += m_i * (ps * (T)rv[k] + vf * (vv[k] - vn * n[k]));
+            if (THERMAL && s.wkind[j] == ESPH_WALL_THERMOSTATIC)
+                dTw += (kap_i + kw) * r * dW * (T_i - Tw) * prcp(r2e);
+        });
+        if (THERMAL) dT += dTw * th_w;
+    }
+    {
+#pragma clang fp contract(off)
+        const double dt = s.dt;
+        double bf[3];
+        if (THERMAL) {
+            const double Tn = S.T[i] + dt * (double)dT;
+            s.Tn[i] = Tn;
+#pragma unroll
+            for (int k = 0; k < D; ++k) bf[k] = -s.g[k] * s.beta * (Tn - s.t0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; ++k) bf[k] = s.g[k];
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            double v = S.v[i * D + k];
+            v += bf[k] * dt;
+            const double dvk = (double)dv[k];
+            v += dvk * dt;
+            s.vn[i * D + k] = v;
+            s.xn[i * D + k] = xi[k] + (v * dt - dvk * (dt * dt) / 2.0);
+        }
+    }
+}
+
+template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH>
+__global__ __launch_bounds__(kBlock) void k_forces(const SceneDev* __restrict__ scenes,
+                                                   const Tile* __restrict__ tiles, int ntiles) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
+        const Tile tile = tiles[t];
+        const SceneDev& s = scenes[tile.scene];
+        const SceneDyn dyn = *s.dyn;
+        if (((s.flags & FLAG_THERMAL) != 0) != THERMAL) continue;
+        for_work<SCAN>(s, dyn, tile, [&](int i) {
+            if (!SCAN && overflowed(s, i)) return;
+            forces_body<D, K, T, THERMAL, SCAN, DEPTH>(s, s.set[dyn.parity], i);
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reconstructScalar!(fluid, :rho_, FF pairs, kernel) (GroupAction/
+// ReconstructValue.jl:22-57): Shepard filter over the step-start pairs with
+// the post-continuity density, in binary64 in every form.
+// ---------------------------------------------------------------------------
+template <int D, int K, bool SCAN, int DEPTH>
+__global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__ scenes,
+                                                    const Tile* __restrict__ tiles, int ntiles) {
+    using KQ = KernelQ<double, K>;
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
+        const Tile tile = tiles[t];
+        const SceneDev& s = scenes[tile.scene];
+        const SceneDyn dyn = *s.dyn;
+        if (s.kreinit <= 0 || (dyn.step + 1) % s.kreinit != 0) continue;
+        for_work<SCAN>(s, dyn, tile, [&](int i) {
+        if (!SCAN && overflowed(s, i)) return;
+        const FluidSet& S = s.set[dyn.parity];
+        const int8_t* kind = S.kind;
+        if (kind && kind[i] != KIND_FLUID) return;
+        double xi[3];
+        load_x<D>(S.x, i, xi);
+        double sw = 0.0, ss = 0.0;
+        struct SRec {
+            double m, rho;
+            int8_t kind;
+        };
+        auto load = [&](int j) {
+            SRec r;
+            r.m = S.m[j];
+            r.rho = S.rho[j];
+            r.kind = kind ? kind[j] : (int8_t)KIND_FLUID;
+            return r;
+        };
+        for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.nbr, s.nnb[i], load,
+                                    [&](int, double d2, const double*, const SRec& o) {
+            if (o.kind != KIND_FLUID) return;  // fluid-fluid pairs only
+            double W, dW;
+            KQ::wdw(sqrt(d2) * s.inv_h, s.kA, s.kB, W, dW);
+            const double rho_j = o.rho;
+            const double a = W * o.m / rho_j;
+            sw += a;
+            ss += a * rho_j;
+        });
+        {
+#pragma clang fp contract(off)
+            const double rho_i = S.rho[i];
+            const double a = s.W0 * S.m[i] / rho_i;
+            sw += a;
+            ss += a * rho_i;
+            s.rhon[i] = ss / sw;
+        }
+        });
+    }
+}
+
+
+template <int D>
+hipError_t launch_continuity_d(const Bucket& b, hipStream_t st) {
+    dim3 blk(kBlock);
+    hipError_t e = hipSuccess;
+    ESPH_DISPATCH_K(b.kernel, {
+        e = launch_both(b, st, [&](auto scan, dim3 g, const Tile* tl, int nt) {
+            constexpr bool S = decltype(scan)::value;
+            ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
+                if (b.prec == ESPH_FP64) hipLaunchKernelGGL((k_continuity<D, K, double, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
+                else hipLaunchKernelGGL((k_continuity<D, K, float, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
+            });
+        });
+    });
+    return e;
+}
+
+template <int D, bool TH>
+hipError_t launch_forces_t(const Bucket& b, hipStream_t st) {
+    dim3 blk(kBlock);
+    hipError_t e = hipSuccess;
+    ESPH_DISPATCH_K(b.kernel, {
+        e = launch_both(b, st, [&](auto scan, dim3 g, const Tile* tl, int nt) {
+            constexpr bool S = decltype(scan)::value;
+            ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
+                if (b.prec == ESPH_FP64) hipLaunchKernelGGL((k_forces<D, K, double, TH, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
+                else hipLaunchKernelGGL((k_forces<D, K, float, TH, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
+            });
+        });
+    });
+    return e;
+}
+
+template <int D>
+hipError_t launch_forces_d(const Bucket& b, hipStream_t st) {
+    if (b.has_thermal) {
+        hipError_t e = launch_forces_t<D, true>(b, st);
+        if (e != hipSuccess) return e;
+    }
+    if (b.has_plain) return launch_forces_t<D, false>(b, st);
+    return hipSuccess;
+}
+
+template <int D>
+hipError_t launch_shepard_d(const Bucket& b, hipStream_t st) {
+    dim3 blk(kBlock);
+    hipError_t e = hipSuccess;
+    ESPH_DISPATCH_K(b.kernel, {
+        e = launch_both(b, st, [&](auto scan, dim3 g, const Tile* tl, int nt) {
+            constexpr bool S = decltype(scan)::value;
+            ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
+                hipLaunchKernelGGL((k_shepard<D, K, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
+            });
+        });
+    });
+    return e;
+}
+
+}  // namespace esph

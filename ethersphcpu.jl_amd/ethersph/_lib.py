@@ -10,7 +10,7 @@ from __future__ import annotations
 import ctypes
 import os
 import weakref
-from ctypes import POINTER, c_char_p, c_double, c_int32, c_int64, c_uint64, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_int64, c_uint64, c_void_p
 
 import numpy as np
 
@@ -90,6 +90,9 @@ def lib() -> ctypes.CDLL:
         "esph_ctx_destroy": ([vp], c_int32),
         "esph_last_error": ([vp], c_char_p),
         "esph_ctx_memory_used": ([vp, POINTER(c_uint64)], c_int32),
+        "esph_ctx_set_tuning": ([vp, c_int32, c_int32], c_int32),
+        "esph_ctx_tuned_depth": ([vp, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_float)],
+                                 c_int32),
         "esph_kernel_eval": ([vp, c_int32, c_int32, c_double, c_int64, DP, DP, DP], c_int32),
         "esph_neighbor_list": ([vp, c_int32, c_double, c_int64, DP, c_int64, DP, POINTER(c_int32), c_int64,
                                 POINTER(c_int64)], c_int32),
@@ -165,6 +168,20 @@ class Context:
     def check(self, rc: int) -> None:
         if rc != ESPH_OK:
             raise EsphError(rc, self._L.esph_last_error(self.handle).decode())
+
+    def set_tuning(self, tune: bool = True, depth: int = 2) -> None:
+        """Gather depth of the pair passes: tuned per scene class at its
+        first step (default), or fixed (1, 2 or 4)."""
+        self.check(self._L.esph_ctx_set_tuning(self.handle, 1 if tune else 0, depth))
+
+    def tuned_depth(self, dim: int, kernel: int, precision: int, thermal_mix: int = 1):
+        """(depth, [ms at depth 1, 2, 4]) of a tuned class, or None."""
+        d = c_int32()
+        ms = (c_float * 3)()
+        rc = self._L.esph_ctx_tuned_depth(self.handle, dim, kernel, precision, thermal_mix, ctypes.byref(d), ms)
+        if rc != ESPH_OK:
+            return None
+        return int(d.value), [float(v) for v in ms]
 
     def memory_used(self) -> int:
         v = c_uint64()

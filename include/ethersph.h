@@ -225,6 +225,19 @@ int esph_ctx_create(int32_t device, uint64_t hbm_budget_bytes, esph_ctx** out);
 int esph_ctx_destroy(esph_ctx* ctx);
 const char* esph_last_error(const esph_ctx* ctx);
 /* Bytes of device memory currently held by scenes of this context. */
+/* Gather-depth tuner (no counterpart in the reference: a tuning knob of this
+   library).  The pair passes keep 1, 2 or 4 neighbour gathers in flight per
+   thread; results are identical at every depth.  mode 1 (default): the first
+   step of each class of scenes (dim, kernel, precision, thermal mix) times
+   the force pass at the three depths on that step's neighbour lists and keeps
+   the fastest; mode 0: the given depth everywhere.  Clears earlier choices;
+   applies to steps launched afterwards. */
+int esph_ctx_set_tuning(esph_ctx* ctx, int32_t mode, int32_t depth);
+/* The depth chosen for a class (thermal_mix: 1 plain scenes, 2 thermal, 3
+   both in one batch bucket) and, if ms3 is not NULL, the force-pass times
+   (ms) measured at depths 1, 2, 4.  ESPH_ERR_STATE if not tuned yet. */
+int esph_ctx_tuned_depth(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision, int32_t thermal_mix,
+                         int32_t* depth, float* ms3);
 int esph_ctx_memory_used(const esph_ctx* ctx, uint64_t* bytes);
 
 /* ---- kernel evaluation (SmoothKernel.jl:37-48 + kernelValue/kernelGradient)
