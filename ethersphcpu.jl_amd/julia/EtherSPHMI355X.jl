@@ -12,15 +12,18 @@
     solve!(fluid, static_velocity_wall, ...)                          Solve/FluidVelocitySolve.jl:104
     solve!(fluid, thermostatic_wall, ..., ::ThermalWeakly...Model)    Solve/FluidThermostaticWallSolve.jl:127
     solve!(fluid, wall, thermostatic_wall, ..., ::ThermalWeakly...)   Solve/FluidCompulsiveThermostaticWallSolve.jl:154
+    solve!(fluid, inlet, outlet, wall, ...; inlet/outlet closures)     Solve/FluidInletOutletWallSolve.jl:167
     kernelValue / kernelGradient                                      SPHKernel/SmoothKernel/*.jl
     CellListMap.neighborlist (as used by findNeighbours!)             Neighbour/Neighbour.jl:88
-  `check` closures cannot run on the device; pass an AABBCheck instead.
+  `check` closures cannot run on the device; pass an AABBCheck instead.  The
+  inlet/outlet closures are planes along an axis (InletOutletPlanes) and the
+  examples' user_defined_function a named profile (BufferProfile).
 =#
 module EtherSPHMI355X
 
 using EtherSPHCPU
 
-export MI355X, AABBCheck, GPUContext, gpu_neighborlist
+export MI355X, AABBCheck, InletOutletPlanes, BufferProfile, GPUContext, gpu_neighborlist
 
 const libesph = joinpath(@__DIR__, "..", "libethersph_hip.so")
 
@@ -33,6 +36,26 @@ MI355X() = MI355X(0)
 struct AABBCheck
     lo::Vector{Float64}
     hi::Vector{Float64}
+end
+
+"The inlet/outlet closures of FluidInletOutletWallSolve.jl:176-180 as planes along `axis` (1-based)."
+struct InletOutletPlanes
+    axis::Int
+    inlet_enter::Float64     # inlet particle with x[axis] > inlet_enter joins the fluid...
+    inlet_shift::Float64     # ...and moves back by inlet_shift
+    outlet_enter::Float64    # fluid particle with x[axis] > outlet_enter joins the outlet
+    outlet_exit::Float64     # outlet particle with x[axis] > outlet_exit is removed
+end
+
+"The examples' user_defined_function: 1 = 2D channel start-up series, 2 = 3D pipe profile."
+struct BufferProfile
+    kind::Int32
+    f::Float64
+    nu::Float64
+    l::Float64
+    center::NTuple{3,Float64}
+    terms::Int32
+    outlet_profile::Bool
 end
 
 # ---- C structures (include/ethersph.h) --------------------------------------
@@ -62,6 +85,18 @@ struct EsphSceneDesc
     domain_set::Int32
     domain_min::NTuple{3,Float64}
     domain_max::NTuple{3,Float64}
+    io_axis::Int32
+    inlet_enter::Float64
+    inlet_shift::Float64
+    outlet_enter::Float64
+    outlet_exit::Float64
+    profile::Int32
+    profile_terms::Int32
+    outlet_profile::Int32
+    profile_f::Float64
+    profile_nu::Float64
+    profile_l::Float64
+    profile_center::NTuple{3,Float64}
 end
 
 struct EsphFluidArrays
@@ -124,18 +159,29 @@ context(dev::MI355X) = get!(() -> GPUContext(dev.device), CONTEXTS, dev.device)
 packvec(ps, f) = reduce(hcat, [getfield(p, f) for p in ps]; init = zeros(Float64, length(getfield(first(ps), f)), 0))
 packsc(ps, f) = Float64[getfield(p, f) for p in ps]
 
-function descriptor(dim, kernel, model, td, solver, check)
+function descriptor(dim, kernel, model, td, solver, check; planes = nothing, profile = nothing, capacity = 0)
     bf = ntuple(k -> k <= dim ? Float64(model.body_force_vec_[k]) : 0.0, 3)
     thermal = model isa ThermalWeaklyCompressibleLiquidModel
     lo = ntuple(k -> (check !== nothing && k <= dim) ? check.lo[k] : -Inf, 3)
     hi = ntuple(k -> (check !== nothing && k <= dim) ? check.hi[k] : Inf, 3)
+    pl = planes === nothing ? InletOutletPlanes(1, 0.0, 0.0, 0.0, 0.0) : planes
+    pr = profile === nothing ? BufferProfile(0, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0), 0, false) : profile
     return EsphSceneDesc(dim, kernel_kind(kernel), kernel.influence_radius_, solver, 0,
                          model.rho_0_, model.c_0_, model.p_0_, model.gamma_, model.mu_0_, bf,
                          thermal ? model.t_0_ : 0.0, thermal ? model.kappa_0_ : 0.0,
                          thermal ? model.cp_ : 0.0, thermal ? model.beta_ : 0.0,
                          td.dt_, td.density_reinitialized_interval_, td.check_interval_,
-                         check === nothing ? 0 : 1, lo, hi, 0, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+                         check === nothing ? 0 : 1, lo, hi, capacity, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0),
+                         pl.axis - 1, pl.inlet_enter, pl.inlet_shift, pl.outlet_enter, pl.outlet_exit,
+                         pr.kind, pr.terms, pr.outlet_profile ? 1 : 0, pr.f, pr.nu, pr.l, pr.center)
 end
+
+fluid_arrays(n, x, v, rho, p, c, m, gap) =
+    EsphFluidArrays(n, pointer(x), pointer(v), pointer(rho), pointer(p), pointer(c), pointer(m), pointer(gap),
+                    C_NULL, C_NULL, C_NULL)
+
+pack_liquid(ps) = (packvec(ps, :x_vec_), packvec(ps, :v_vec_), packsc(ps, :rho_), packsc(ps, :p_), packsc(ps, :c_),
+                   packsc(ps, :mass_), packsc(ps, :gap_))
 
 function run_gpu!(fluid, walls, kernel, model, td, solver, check, dev::MI355X)
     ctx = context(dev)
@@ -227,6 +273,70 @@ function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, wall::Abstra
                             td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
                             check::Union{Nothing,AABBCheck} = nothing)
     run_gpu!(fluid, [wall, thermo], kernel, model, td, Int32(3), check, dev)
+end
+
+# FluidInletOutletWallSolve.jl:167-181: the groups change size (inlet copies join the
+# fluid, leavers join the outlet, expired outlet particles are removed); the
+# caller's vectors are refilled in reference order.
+function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, inlet::AbstractVector{<:FluidParticle},
+                            outlet::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:WallParticle},
+                            kernel::SmoothKernel, model::WeaklyCompressibleLiquidModel,
+                            td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
+                            inlet_outlet::InletOutletPlanes, user_defined_function::Union{Nothing,BufferProfile} = nothing)
+    ctx = context(dev)
+    dim = length(first(fluid).x_vec_)
+    desc = Ref(descriptor(dim, kernel, model, td, Int32(4), nothing; planes = inlet_outlet,
+                          profile = user_defined_function))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    chk(ctx, ccall((:esph_scene_create, libesph), Cint, (Ptr{Cvoid}, Ptr{EsphSceneDesc}, Ptr{Ptr{Cvoid}}),
+                     ctx.handle, desc, h))
+    scene = h[]
+    try
+        wx = packvec(wall, :x_vec_); wn = packvec(wall, :normal_vec_); wg = packsc(wall, :gap_)
+        wz = zeros(dim, length(wall)); wz1 = zeros(length(wall))
+        GC.@preserve wx wn wg wz wz1 begin
+            wa = [EsphWallArrays(wall_kind(wall), length(wall), pointer(wx), pointer(wn), pointer(wg), pointer(wz),
+                                 pointer(wz1), pointer(wz1))]
+            chk(ctx, ccall((:esph_scene_set_walls, libesph), Cint, (Ptr{Cvoid}, Int32, Ptr{EsphWallArrays}),
+                             scene, 1, wa))
+        end
+        bi = pack_liquid(inlet); bo = pack_liquid(outlet); fl = pack_liquid(fluid)
+        GC.@preserve bi bo fl begin
+            ia = Ref(fluid_arrays(length(inlet), bi...)); oa = Ref(fluid_arrays(length(outlet), bo...))
+            chk(ctx, ccall((:esph_scene_set_buffers, libesph), Cint,
+                             (Ptr{Cvoid}, Ptr{EsphFluidArrays}, Ptr{EsphFluidArrays}), scene, ia, oa))
+            fa = Ref(fluid_arrays(length(fluid), fl...))
+            chk(ctx, ccall((:esph_scene_set_fluid, libesph), Cint, (Ptr{Cvoid}, Ptr{EsphFluidArrays}), scene, fa))
+        end
+        chk(ctx, ccall((:esph_scene_step, libesph), Cint, (Ptr{Cvoid}, Int64), scene, td.total_step_))
+        chk(ctx, ccall((:esph_scene_sync, libesph), Cint, (Ptr{Cvoid},), scene))
+        for (group, ps) in ((0, fluid), (1, inlet), (2, outlet))
+            n = Ref{Int64}(0)
+            chk(ctx, ccall((:esph_scene_group_count, libesph), Cint, (Ptr{Cvoid}, Int32, Ptr{Int64}), scene, group, n))
+            get(field, w) = begin
+                out = Vector{Float64}(undef, max(n[] * w, 1))
+                wr = Ref{Int64}(0)
+                chk(ctx, ccall((:esph_scene_get_group, libesph), Cint,
+                                 (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Int64, Ptr{Int64}),
+                                 scene, group, field, out, length(out), wr))
+                out[1:wr[]]
+            end
+            proto = first(ps)
+            while length(ps) < n[]
+                push!(ps, deepcopy(proto))
+            end
+            resize!(ps, n[])
+            X = reshape(get(0, dim), dim, :); V = reshape(get(1, dim), dim, :)
+            R = get(2, 1); P = get(3, 1); C = get(4, 1); M = get(5, 1); G = get(6, 1)
+            for (i, q) in enumerate(ps)
+                q.x_vec_ .= X[:, i]; q.v_vec_ .= V[:, i]
+                q.rho_ = R[i]; q.p_ = P[i]; q.c_ = C[i]; q.mass_ = M[i]; q.gap_ = G[i]
+            end
+        end
+    finally
+        ccall((:esph_scene_destroy, libesph), Cint, (Ptr{Cvoid},), scene)
+    end
+    return nothing
 end
 
 # ---- kernels and neighbour lists on the device -----------------------------------------
