@@ -564,15 +564,17 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         {
             std::vector<double> A(4 * (size_t)std::max(nw, 1), 0.0), B(4 * (size_t)std::max(nw, 1), 0.0);
             std::vector<double> K(std::max(nw, 1), 0.0);
+            // float records hold T - t0 (see the fluid records)
+            const double Toff = f32 ? s->d.t_0 : 0.0;
             for (int p = 0; p < nw; ++p) {
                 double* a = &A[4 * (size_t)p];
                 double* b = &B[4 * (size_t)p];
                 if (D == 3) {
                     a[0] = ws.n_[3 * p]; a[1] = ws.n_[3 * p + 1]; a[2] = ws.n_[3 * p + 2]; a[3] = ws.gap[p];
-                    b[0] = ws.v[3 * p]; b[1] = ws.v[3 * p + 1]; b[2] = ws.v[3 * p + 2]; b[3] = ws.T[p];
+                    b[0] = ws.v[3 * p]; b[1] = ws.v[3 * p + 1]; b[2] = ws.v[3 * p + 2]; b[3] = ws.T[p] - Toff;
                     K[p] = ws.kappa[p];
                 } else {
-                    a[0] = ws.n_[2 * p]; a[1] = ws.n_[2 * p + 1]; a[2] = ws.gap[p]; a[3] = ws.T[p];
+                    a[0] = ws.n_[2 * p]; a[1] = ws.n_[2 * p + 1]; a[2] = ws.gap[p]; a[3] = ws.T[p] - Toff;
                     b[0] = ws.v[2 * p]; b[1] = ws.v[2 * p + 1]; b[2] = ws.kappa[p]; b[3] = 0.0;
                 }
             }

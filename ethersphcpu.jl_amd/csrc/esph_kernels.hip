@@ -366,7 +366,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
             O.cp[p] = S.cp[src];
         }
         if (s.prec == ESPH_FP32_BULK) {
-            store_sorted_rec<float>(O, D, p, v, m, gap, Tt, kap, thermal);
+            // float records hold T - t0: the conduction terms use only
+            // differences, which keep their precision near the reference
+            // temperature this way
+            store_sorted_rec<float>(O, D, p, v, m, gap, Tt - s.t0, kap, thermal);
             if (kd != KIND_FLUID) {  // buffers never pass the continuity pass
                 float* B = static_cast<float*>(O.B) + 4 * (size_t)p;
                 B[0] = (float)rho;
