@@ -311,7 +311,7 @@ static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) 
     // cells half of R wide (a 5-cell stencil per axis: about 300 candidates per
     // particle in 3D instead of about 500 with R-wide cells), widened when the
     // grid would exceed 2^27 cells
-    double cs = s->d.influence_radius * 1.0001 / 2.0;
+    double cs = hd.Rv * 1.0001 / 2.0;
     for (;;) {
         int64_t ncell = 1;
         for (int k = 0; k < 3; ++k) {
@@ -326,8 +326,8 @@ static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) 
     }
     hd.inv_cs = 1.0 / cs;
     hd.cs = cs;
-    hd.stencil = (int32_t)std::ceil(s->d.influence_radius / cs);
-    if ((double)hd.stencil * cs <= s->d.influence_radius) hd.stencil += 1;
+    hd.stencil = (int32_t)std::ceil(hd.Rv / cs);
+    if ((double)hd.stencil * cs <= hd.Rv) hd.stencil += 1;
 }
 
 static int cell_of_host(const SceneDev& hd, int D, const double* x) {
@@ -385,6 +385,17 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.kcheck = d.check_interval;
     hd.R = d.influence_radius;
     hd.R2 = d.influence_radius * d.influence_radius;
+    {
+        // candidate lists reused across steps (not for the inlet/outlet
+        // solver, whose migrations reorder the particles every step)
+        const double skin = d.verlet_skin == 0.0 ? kDefaultSkin : d.verlet_skin;
+        hd.verlet = (skin > 0.0 && !io) ? 1 : 0;
+        const double sk = hd.verlet ? skin : 0.0;
+        hd.Rv = d.influence_radius * (1.0 + sk);
+        hd.Rv2 = hd.Rv * hd.Rv;
+        const double half = 0.5 * sk * d.influence_radius;
+        hd.disp2_limit = half * half * (1.0 - 1e-9);
+    }
     hd.h = s->kh;
     hd.sigma = s->ksigma;
     hd.W0 = s->ksigma;
@@ -417,7 +428,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     // particles' spacing (gap of the first particle) plus 16, multiple of 8
     {
         double spacing = (f->n > 0 && f->gap && f->gap[0] > 0) ? f->gap[0] : d.influence_radius / 1.5;
-        const double a = d.influence_radius / spacing;
+        const double a = hd.Rv / spacing;
         const double est = D == 2 ? 3.14159265358979 * a * a : 4.18879020478639 * a * a * a;
         int k = (int)std::ceil(1.5 * est + 16.0);
         k = std::min(std::max((k + 15) / 16 * 16, 16), 1024);  // whole 16-entry chunks
@@ -435,7 +446,8 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
                    (size_t)nw * (3 * D * G + 3 * G + 2 * I + 9 * P) + (size_t)(hd.ncell + 1) * I +
                    sizeof(SceneDyn) + sizeof(SceneDev) +
                    ((size_t)(cap + kBlock - 1) / kBlock + (size_t)hd.cell_tiles) * sizeof(Tile) +
-                   (size_t)cap * I * (hd.kff + hd.kfw + 2) + 2 * (size_t)cap * 4 * F;
+                   (size_t)cap * I * (hd.kff * (hd.verlet ? 2 : 1) + hd.kfw + 4) + 2 * (size_t)cap * 4 * F +
+                   (size_t)cap * D * G;
     if (ctx->budget && ctx->used + bytes > ctx->budget)
         return fail(ctx, ESPH_ERR_BUDGET, "scene needs " + std::to_string(bytes) + " bytes; budget left " +
                                               std::to_string(ctx->budget - ctx->used));
@@ -474,8 +486,11 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.count = (int32_t*)dal(I * (hd.ncell + 1));
     hd.cell_start = (int32_t*)dal(I * (hd.ncell + 2));
     hd.bsum = (int32_t*)dal(I * hd.cell_tiles);
-    hd.nbr = (int32_t*)dal(I * (size_t)cap * hd.kff);
-    hd.nnb = (int32_t*)dal(I * (size_t)cap);
+    hd.vnbr = (int32_t*)dal(I * (size_t)cap * hd.kff);
+    hd.nvnb = (int32_t*)dal(I * (size_t)cap);
+    hd.nbr = hd.verlet ? (int32_t*)dal(I * (size_t)cap * hd.kff) : hd.vnbr;
+    hd.nnb = hd.verlet ? (int32_t*)dal(I * (size_t)cap) : hd.nvnb;
+    hd.xref = (double*)dal(G * (size_t)cap * D);
     hd.wnbr = (int32_t*)dal(I * (size_t)cap * std::max(hd.kfw, 1));
     hd.nnbw = (int32_t*)dal(I * (size_t)cap);
     hd.ovf = (int32_t*)dal(I * (size_t)cap);
@@ -1059,8 +1074,9 @@ extern "C" int esph_scene_phase_times(esph_scene* s, double* ms5, int64_t* steps
     return ESPH_OK;
 }
 
-extern "C" int esph_scene_stats(esph_scene* s, int64_t* out6) {
-    if (!s || !out6) return ESPH_ERR_ARG;
+extern "C" int esph_scene_stats(esph_scene* s, int64_t* out8) {
+    if (!s || !out8) return ESPH_ERR_ARG;
+    int64_t* out6 = out8;
     if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
     SceneDyn dyn;
     int rc = read_dyn(s, &dyn);
@@ -1071,6 +1087,8 @@ extern "C" int esph_scene_stats(esph_scene* s, int64_t* out6) {
     out6[3] = s->h.ncell;
     out6[4] = dyn.outside;
     out6[5] = dyn.err;
+    out8[6] = dyn.nsorts;
+    out8[7] = s->h.verlet ? s->h.kff : 0;
     return ESPH_OK;
 }
 

@@ -55,7 +55,10 @@ __device__ __forceinline__ void for_each_row(const SceneDev& s, const int32_t* _
     const int m = s.stencil;
     const double cs = s.cs;
     const double slack = 1e-7 * cs;
-    const double lim = s.R2 * (1.0 + 1e-6);
+    // trimmed against the candidate radius Rv: between list rebuilds the
+    // particles sit up to (Rv - R) / 2 away from the cells they were sorted
+    // into, and Rv covers that
+    const double lim = s.Rv2 * (1.0 + 1e-6);
     double p[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) p[k] = k < D ? xi[k] - (s.lo[k] + (double)c[k] * cs) : 0.0;
@@ -127,12 +130,13 @@ __device__ __forceinline__ size_t list_at(size_t cap, int k, int i) {
     return (((size_t)(k >> 2) * cap + (size_t)i) << 2) + (size_t)(k & 3);
 }
 
-// Candidates of particle i over the adjacent cells, tested first in float on
-// the grid-relative positions xf: a candidate whose float distance is clearly
-// inside or clearly outside R is decided there; only those within the
-// rounding band tau of R take the FP64 rule of for_each_neighbour.  tau bounds
-// the float error: 2^-21 (|xr_i|_inf + 2 R), so the decisions equal the FP64
-// rule's.  Calls f(j) for every neighbour, in cell order.
+// Candidates of particle i within the candidate radius Rv over the adjacent
+// cells, tested first in float on the grid-relative positions xf: a candidate
+// whose float distance is clearly inside or clearly outside Rv is decided
+// there; only those within the rounding band tau of Rv take the FP64 rule
+// (d^2 <= Rv^2).  tau bounds the float error: 2^-21 (|xr_i|_inf + 2 Rv), so the
+// decisions equal the FP64 rule's.  Calls f(j) for every candidate, in cell
+// order.  Used right after a sort, when xf and the cells are current.
 template <int D, typename F>
 __device__ __forceinline__ void for_each_candidate_f32(const SceneDev& s, const int32_t* __restrict__ cstart,
                                                        const void* __restrict__ xfall,
@@ -140,11 +144,11 @@ __device__ __forceinline__ void for_each_candidate_f32(const SceneDev& s, const 
                                                        const int c[3], int self, F&& f) {
     float xr[3] = {(float)(xi[0] - s.lo[0]), (float)(xi[1] - s.lo[1]), D == 3 ? (float)(xi[2] - s.lo[2]) : 0.0f};
     const double amax = fmax(fabs((double)xr[0]), fmax(fabs((double)xr[1]), fabs((double)xr[2])));
-    const double tau = ldexp(amax + 2.0 * s.R, -21);
-    const double rin = s.R - tau, rout = s.R + tau;
+    const double tau = ldexp(amax + 2.0 * s.Rv, -21);
+    const double rin = s.Rv - tau, rout = s.Rv + tau;
     const float lo2 = rin > 0.0 ? (float)(rin * rin) * (1.0f - 0x1p-22f) : -1.0f;
     const float hi2 = (float)(rout * rout) * (1.0f + 0x1p-22f);
-    const double R2 = s.R2;
+    const double R2 = s.Rv2;
     for_each_row<D>(s, cstart, xi, c, [&](int j0, int j1) {
         for (int j = j0; j < j1; j += 4) {
             float d2f[4];
@@ -318,8 +322,27 @@ __device__ __forceinline__ void for_work(const SceneDev& s, const SceneDyn& dyn,
 }
 
 __device__ __forceinline__ bool overflowed(const SceneDev& s, int i) {
-    return s.nnb[i] > s.kff || (s.nw > 0 && s.nnbw[i] > s.kfw);
+    return s.nvnb[i] > s.kff || (s.nw > 0 && s.nnbw[i] > s.kfw);
 }
+
+// Appends j to particle i's chunked list through a 4-entry register buffer
+// (call list_flush once after the last entry).
+struct ListWriter {
+    int4 buf = make_int4(0, 0, 0, 0);
+    int cnt = 0;
+    __device__ __forceinline__ void add(int32_t* __restrict__ list, size_t cap, int i, int j) {
+        const int r = cnt & 3;
+        buf.x = r == 0 ? j : buf.x;
+        buf.y = r == 1 ? j : buf.y;
+        buf.z = r == 2 ? j : buf.z;
+        buf.w = r == 3 ? j : buf.w;
+        if (r == 3) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
+        ++cnt;
+    }
+    __device__ __forceinline__ void flush(int32_t* __restrict__ list, size_t cap, int i) {
+        if ((cnt & 3) != 0) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
+    }
+};
 
 // Velocity record of a neighbour (continuity pass).
 template <typename T>

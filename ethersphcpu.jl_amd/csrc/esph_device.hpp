@@ -28,6 +28,9 @@ enum : int8_t { KIND_FLUID = 0, KIND_INLET = 1, KIND_OUTLET = 2 };
 
 // Per-scene state that changes every step and lives in device memory, so a
 // captured step graph can be replayed without the host.
+// Candidate-list skin (fraction of R) when the descriptor leaves it at 0.
+constexpr double kDefaultSkin = 0.1;
+
 struct SceneDyn {
     int64_t step;    // completed steps (the reference's `step` of the last one)
     int32_t parity;  // which FluidSet is current
@@ -40,6 +43,10 @@ struct SceneDyn {
     int32_t next_fluid_id;   // next sequence number of the fluid group
     int32_t next_outlet_id;  // next sequence number of the outlet group
     int32_t outside;   // particles more than one cell outside the grid at the last sort
+    int32_t rebuild;   // 1: the next continuity pass rebuilds the candidate lists (a sort just ran)
+    int32_t sorted;    // this step's decision: sort (and rebuild) or commit in place
+    uint32_t disp2_bits;  // float bits of this step's largest squared displacement from xref
+    int32_t nsorts;       // sorts (list rebuilds) so far, the upload's included
 };
 
 // One sorted copy of the fluid state.  Masters are binary64 SoA arrays (state
@@ -84,6 +91,12 @@ struct SceneDev {
     // kernel (SmoothKernel.jl:37-48); kA, kB: factors of W(q) and dW/dr(q)
     // in the q-forms below, inv_h = 1 / h
     double R, R2, h, sigma, W0;
+    // candidate radius Rv = R (1 + skin): the grid, the candidate lists and
+    // their reuse across steps while no particle has moved skin R / 2 since
+    // the lists were built (disp2_limit); verlet = 0: rebuilt every step
+    double Rv, Rv2, disp2_limit;
+    int32_t verlet;
+    int32_t pad3;
     double inv_h, kA, kB;
     // model (LiquidModel.jl:10-66)
     double rho0, c0, p0, b, nu0, t0, cp, beta, dt;
@@ -114,10 +127,13 @@ struct SceneDev {
     int32_t pad0;
     // per-step neighbour lists in 4-entry chunks: entry k of particle i at
     // [((k / 4) * cap + i) * 4 + k % 4]
-    int32_t* nbr;
-    int32_t* nnb;         // fluid-fluid count per particle
-    int32_t* wnbr;
-    int32_t* nnbw;        // fluid-wall count per particle
+    int32_t* nbr;         // this step's fluid-fluid neighbours (exact rule), when verlet
+    int32_t* nnb;         // their count per particle
+    int32_t* vnbr;        // fluid-fluid candidates within Rv (the neighbours themselves when !verlet)
+    int32_t* nvnb;        // their count per particle (overflow when > kff)
+    int32_t* wnbr;        // fluid-wall candidates within Rv (filtered by the exact rule when used)
+    int32_t* nnbw;        // their count per particle
+    double* xref;         // positions when the candidate lists were built
     int32_t* ovf;         // particles whose lists overflowed this step (they scan their cells)
     // walls: static, sorted by cell once
     double* wx;      // nw*dim

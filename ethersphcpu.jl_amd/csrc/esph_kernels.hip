@@ -26,6 +26,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_clear(const SceneDev* __restric
         const int t = tile_index(tb, ntiles);
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
+        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
 #pragma unroll
         for (int q = 0; q < kScanItems; ++q) {
             const int c = tile.begin + q * kBlock + (int)threadIdx.x;
@@ -79,6 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
         const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
+        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
         const SceneDyn dyn = *s.dyn;
         const int i = tile.begin + (int)threadIdx.x;
         if (i >= dyn.n_live) continue;
@@ -172,6 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
         const int t = tile_index(tb, ntiles);
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
+        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
         const int c0 = tile.begin + (int)threadIdx.x * kScanItems;
         int v[kScanItems];
         int sum = 0;
@@ -195,6 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
 __global__ __launch_bounds__(kBlock) void k_sort_scan2(const SceneDev* __restrict__ scenes, int nscenes) {
     for (int sc = blockIdx.x; sc < nscenes; sc += gridDim.x) {
         const SceneDev& s = scenes[sc];
+        if (!s.dyn->sorted) continue;
         int carry = 0;
         for (int base = 0; base < s.cell_tiles; base += kBlock) {
             const int q = base + (int)threadIdx.x;
@@ -214,6 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan3(const SceneDev* __restric
         const int t = tile_index(tb, ntiles);
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
+        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
         const int off = s.bsum[tile.begin / kCellTile];
 #pragma unroll
         for (int q = 0; q < kScanItems; ++q) {
@@ -233,6 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_place(const SceneDev* __restric
         const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
+        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
         const int i = tile.begin + (int)threadIdx.x;
         const SceneDyn dyn = *s.dyn;
         if (i < dyn.n_live) {
@@ -283,6 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
         const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
+        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
         const SceneDyn dyn = *s.dyn;
         const bool io = mode == 0 && (s.flags & FLAG_IO);
         const int ns = io ? spawn_count(s, dyn) : 0;
@@ -346,6 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
                                                         (float)(x[2] - s.lo[2]), 0.0f);
         else
             static_cast<float2*>(O.xf)[p] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
+        for (int k = 0; k < D; ++k) s.xref[p * D + k] = x[k];  // the candidate lists are rebuilt from here
         const bool rho_new = mode == 1 || (shepard_due(s, dyn) && src_fluid && !spawn);
         const double rho = rho_new ? s.rhon[src] : S.rho[src];
         const double pp = S.p[src];
@@ -386,21 +394,90 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
     }
 }
 
+// Whether this step ends with a sort: always at upload, when the candidate
+// lists are rebuilt every step, on check steps, for the inlet/outlet solver,
+// and when some particle has moved (Rv - R) / 2 since the lists were built
+// (or a position is not finite).  Otherwise the step commits in place.
+__global__ void k_sort_decide(const SceneDev* __restrict__ scenes, int nscenes, int mode) {
+    const int sc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc >= nscenes) return;
+    const SceneDev& s = scenes[sc];
+    SceneDyn* d = s.dyn;
+    const float disp2 = __uint_as_float(d->disp2_bits);
+    const bool moved = !((double)disp2 <= s.disp2_limit);
+    d->sorted = (mode != 0 || !s.verlet || (s.flags & FLAG_IO) || check_due(s, *d) || moved) ? 1 : 0;
+}
+
+// A step without sort: the step's outputs replace the current set in place
+// (same order and cells, so the candidate lists stay valid).
+template <typename T>
+__device__ __forceinline__ void commit_records(const SceneDev& s, const FluidSet& S, int D, int i, const double* v,
+                                               double Tt, bool thermal) {
+    const double kap = thermal ? S.kappa[i] : 0.0;
+    // float records hold T - t0 (see k_sort_scatter)
+    store_sorted_rec<T>(S, D, i, v, S.m[i], S.gap[i], sizeof(T) == 4 ? Tt - s.t0 : Tt, kap, thermal);
+}
+
+__global__ __launch_bounds__(kBlock) void k_commit(const SceneDev* __restrict__ scenes,
+                                                   const Tile* __restrict__ tiles, int ntiles) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
+        const Tile tile = tiles[t];
+        const SceneDev& s = scenes[tile.scene];
+        const SceneDyn dyn = *s.dyn;
+        if (dyn.sorted) continue;
+        const int i = tile.begin + (int)threadIdx.x;
+        if (i >= dyn.n_live) continue;
+        const FluidSet& S = s.set[dyn.parity];
+        const int D = s.dim;
+        double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < D; ++k) {
+            x[k] = s.xn[i * D + k];
+            v[k] = s.vn[i * D + k];
+            S.x[i * D + k] = x[k];
+            S.v[i * D + k] = v[k];
+        }
+        if (D == 3)
+            static_cast<float4*>(S.xf)[i] = make_float4((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]),
+                                                        (float)(x[2] - s.lo[2]), 0.0f);
+        else
+            static_cast<float2*>(S.xf)[i] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
+        if (shepard_due(s, dyn)) S.rho[i] = s.rhon[i];
+        const bool thermal = (s.flags & FLAG_THERMAL) != 0;
+        double Tt = 0.0;
+        if (thermal) {
+            Tt = s.Tn[i];
+            S.T[i] = Tt;
+        }
+        if (s.prec == ESPH_FP32_BULK)
+            commit_records<float>(s, S, D, i, v, Tt, thermal);
+        else
+            commit_records<double>(s, S, D, i, v, Tt, thermal);
+    }
+}
+
 __global__ void k_sort_finish(const SceneDev* __restrict__ scenes, int nscenes, int mode) {
     const int sc = blockIdx.x * blockDim.x + threadIdx.x;
     if (sc >= nscenes) return;
     const SceneDev& s = scenes[sc];
     SceneDyn* d = s.dyn;
-    if (mode == 0 && (s.flags & FLAG_IO)) {
-        const int room = s.cap - d->n_live;  // n_live is still the step's input count here
-        d->next_fluid_id += d->nspawn < room ? d->nspawn : room;
-        d->next_outlet_id += d->nleave;
+    if (d->sorted) {
+        if (mode == 0 && (s.flags & FLAG_IO)) {
+            const int room = s.cap - d->n_live;  // n_live is still the step's input count here
+            d->next_fluid_id += d->nspawn < room ? d->nspawn : room;
+            d->next_outlet_id += d->nleave;
+        }
+        d->n_live = s.cell_start[s.ncell];
+        d->parity ^= 1;
+        d->rebuild = 1;
+        d->nsorts += 1;
+    } else {
+        d->rebuild = 0;
     }
-    d->n_live = s.cell_start[s.ncell];
-    d->parity ^= 1;
     d->novf = 0;
     d->nspawn = 0;
     d->nleave = 0;
+    d->disp2_bits = 0;
     if (mode == 0) d->step += 1;
 }
 
@@ -517,6 +594,7 @@ hipError_t launch_shepard(const Bucket& b, hipStream_t st) {
 
 hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st) {
     dim3 blk(kBlock);
+    hipLaunchKernelGGL(k_sort_decide, dim3((p.nscenes + 63) / 64), dim3(64), 0, st, p.scenes, p.nscenes, mode);
     hipLaunchKernelGGL(k_sort_clear, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
     hipLaunchKernelGGL(k_sort_bin, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_scan1, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
@@ -524,6 +602,7 @@ hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st) {
     hipLaunchKernelGGL(k_sort_scan3, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
     hipLaunchKernelGGL(k_sort_place, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_scatter, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
+    hipLaunchKernelGGL(k_commit, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles);
     hipLaunchKernelGGL(k_sort_finish, dim3((p.nscenes + 63) / 64), dim3(64), 0, st, p.scenes, p.nscenes, mode);
     return hipGetLastError();
 }

@@ -29,6 +29,7 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
             // inlet/outlet buffers keep rho and p (drho is reset,
             // FluidInletOutletWallSolve.jl:91-99) and need no lists
             s.nnb[i] = 0;
+            s.nvnb[i] = 0;
             if (s.nw > 0) s.nnbw[i] = 0;
             return;
         }
@@ -39,31 +40,38 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
         load_A<D, T>(S.A, i, vi, m_i, gap_i);
         int cnt = 0;
         if (!SCAN) {
-            int c[3];
-            cell_of<D>(s, xi, c);
-            cnt = build_list<D>(s, s.cell_start, S.xf, S.x, xi, c, i, i, s.nbr, s.kff);
-            s.nnb[i] = cnt;
-            bool over = cnt > s.kff;
-            if (s.nw > 0) {
-                const int cw = build_list<D>(s, s.wcell_start, s.wxf, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
-                s.nnbw[i] = cw;
-                over = over || cw > s.kfw;
+            if (dyn.rebuild || !s.verlet) {
+                // right after a sort: candidate lists within Rv from the cells
+                int c[3];
+                cell_of<D>(s, xi, c);
+                cnt = build_list<D>(s, s.cell_start, S.xf, S.x, xi, c, i, i, s.vnbr, s.kff);
+                s.nvnb[i] = cnt;
+                if (s.nw > 0) s.nnbw[i] = build_list<D>(s, s.wcell_start, s.wxf, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
+            } else {
+                cnt = s.nvnb[i];
             }
-            if (over) {
+            if (overflowed(s, i)) {
                 // finished by the SCAN instance of this kernel
                 s.ovf[atomicAdd(&s.dyn->novf, 1)] = i;
                 atomicAdd(&s.dyn->overflow, 1);
                 return;
             }
         }
+        // with reused candidate lists the step's neighbours (exact rule) are
+        // written to nbr for the force and Shepard passes
+        const bool exact_list = !SCAN && s.verlet;
+        const double R2 = s.R2;
+        ListWriter lw;
         T drho = T(0);
         auto load = [&](int j) {
             ARec<T> a;
             load_A<D, T>(S.A, j, a.v, a.m, a.gap);
             return a;
         };
-        for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.nbr, cnt, load,
-                                    [&](int, double d2, const double* rv, const ARec<T>& a) {
+        for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
+                                    [&](int j, double d2, const double* rv, const ARec<T>& a) {
+            if (!(d2 <= R2)) return;
+            if (exact_list) lw.add(s.nbr, (size_t)s.cap, i, j);
             const T* vj = a.v;
             const T m_j = a.m;
             const T r = psqrt((T)d2);
@@ -76,6 +84,10 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
             for (int k = 0; k < D; ++k) dot += (vi[k] - vj[k]) * (T)rv[k];
             drho += m_j * sg * dot;
         });
+        if (exact_list) {
+            lw.flush(s.nbr, (size_t)s.cap, i);
+            s.nnb[i] = lw.cnt;
+        }
         {
 #pragma clang fp contract(off)
             double rho = S.rho[i];
@@ -127,12 +139,14 @@ __device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S
     }
 }
 
+// Returns the particle's squared displacement from its position when the
+// candidate lists were built (0 when the lists are rebuilt every step).
 template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH>
-__device__ __forceinline__ void forces_body(const SceneDev& s, const FluidSet& S, int i) {
+__device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet& S, int i) {
     using KQ = KernelQ<T, K>;
     if (S.kind && S.kind[i] != KIND_FLUID) {
         buffer_move<D>(s, S, i);
-        return;
+        return 0.0;
     }
     const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
     const T eta2 = (T)(0.01 * (s.h * s.h));
@@ -196,8 +210,10 @@ __device__ __forceinline__ void forces_body(const SceneDev& s, const FluidSet& S
             w.kw = (D == 3 && THERMAL) ? ld<T>(s.wkap, j) : T(0);
             return w;
         };
+        const double R2 = s.R2;
         for_pairs_d<D, SCAN, DEPTH>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], wload,
                                     [&](int j, double d2, const double* rv, const WRec& w) {
+            if (!(d2 <= R2)) return;  // candidates within Rv: the exact rule
             const V wa = w.wa;
             const V wb = w.wb;
             T n[3], vw[3], gap_j, Tw, kw;
@@ -271,6 +287,7 @@ __device__ __forceinline__ void forces_body(const SceneDev& s, const FluidSet& S
 #pragma unroll
             for (int k = 0; k < D; ++k) bf[k] = s.g[k];
         }
+        double disp2 = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             double v = S.v[i * D + k];
@@ -278,8 +295,14 @@ __device__ __forceinline__ void forces_body(const SceneDev& s, const FluidSet& S
             const double dvk = (double)dv[k];
             v += dvk * dt;
             s.vn[i * D + k] = v;
-            s.xn[i * D + k] = xi[k] + (v * dt - dvk * (dt * dt) / 2.0);
+            const double xk = xi[k] + (v * dt - dvk * (dt * dt) / 2.0);
+            s.xn[i * D + k] = xk;
+            if (s.verlet) {
+                const double e = xk - s.xref[i * D + k];
+                disp2 += e * e;
+            }
         }
+        return disp2;
     }
 }
 
@@ -292,10 +315,28 @@ __global__ __launch_bounds__(kBlock) void k_forces(const SceneDev* __restrict__ 
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
         if (((s.flags & FLAG_THERMAL) != 0) != THERMAL) continue;
+        double dmax = 0.0;
         for_work<SCAN>(s, dyn, tile, [&](int i) {
             if (!SCAN && overflowed(s, i)) return;
-            forces_body<D, K, T, THERMAL, SCAN, DEPTH>(s, s.set[dyn.parity], i);
+            const double d2 = forces_body<D, K, T, THERMAL, SCAN, DEPTH>(s, s.set[dyn.parity], i);
+            dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
         });
+        if (s.verlet) {
+            // wavefront maximum of the squared displacements (rounded up to
+            // float; float bits of non-negative values and NaN order as
+            // unsigned integers), then one atomic per wavefront only when it
+            // raises the scene's running maximum: a contended atomic per
+            // wavefront would serialise the whole pass on one address
+            uint32_t bits = __float_as_uint(__double2float_ru(dmax));
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint32_t other = __shfl_xor(bits, o, 64);
+                bits = other > bits ? other : bits;
+            }
+            if ((threadIdx.x & 63) == 0 && bits != 0 &&
+                bits > __hip_atomic_load(&s.dyn->disp2_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(&s.dyn->disp2_bits, bits);
+        }
     }
 }
 

@@ -50,6 +50,7 @@ class SceneDesc(ctypes.Structure):
         ("io_axis", c_int32), ("inlet_enter", c_double), ("inlet_shift", c_double), ("outlet_enter", c_double),
         ("outlet_exit", c_double), ("profile", c_int32), ("profile_terms", c_int32), ("outlet_profile", c_int32),
         ("profile_f", c_double), ("profile_nu", c_double), ("profile_l", c_double), ("profile_center", D3),
+        ("verlet_skin", c_double),
     ]
 
 

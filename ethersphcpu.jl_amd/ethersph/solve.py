@@ -307,10 +307,10 @@ class Scene:
         return {"sort": ms[0], "continuity": ms[1], "forces": ms[2], "shepard": ms[3], "steps": n.value}
 
     def stats(self) -> dict:
-        v = (c_int64 * 6)()
+        v = (c_int64 * 8)()
         self.ctx.check(self.L.esph_scene_stats(self.handle, v))
         return {"list_overflow": v[0], "list_width_ff": v[1], "list_width_fw": v[2], "cells": v[3],
-                "outside_grid": v[4], "error_bits": v[5]}
+                "outside_grid": v[4], "error_bits": v[5], "sorts": v[6], "reused_lists": v[7] > 0}
 
     def enable_graph(self, on: bool = True) -> None:
         self.ctx.check(self.L.esph_scene_enable_graph(self.handle, int(on)))

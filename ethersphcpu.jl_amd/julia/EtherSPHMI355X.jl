@@ -97,6 +97,7 @@ struct EsphSceneDesc
     profile_nu::Float64
     profile_l::Float64
     profile_center::NTuple{3,Float64}
+    verlet_skin::Float64
 end
 
 struct EsphFluidArrays
@@ -173,7 +174,7 @@ function descriptor(dim, kernel, model, td, solver, check; planes = nothing, pro
                          td.dt_, td.density_reinitialized_interval_, td.check_interval_,
                          check === nothing ? 0 : 1, lo, hi, capacity, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0),
                          pl.axis - 1, pl.inlet_enter, pl.inlet_shift, pl.outlet_enter, pl.outlet_exit,
-                         pr.kind, pr.terms, pr.outlet_profile ? 1 : 0, pr.f, pr.nu, pr.l, pr.center)
+                         pr.kind, pr.terms, pr.outlet_profile ? 1 : 0, pr.f, pr.nu, pr.l, pr.center, 0.0)
 end
 
 fluid_arrays(n, x, v, rho, p, c, m, gap) =

@@ -187,6 +187,13 @@ typedef struct esph_scene_desc {
     double profile_nu;
     double profile_l;         /* channel width (2D) or pipe radius (3D) */
     double profile_center[3];
+    /* candidate-list skin as a fraction of R: the lists hold every particle
+       within R (1 + skin) and are reused, filtered by the exact rule, until
+       some particle has moved skin R / 2 (or a check or migration step
+       needs a sort).  0: the library default (0.1); negative: the lists are
+       rebuilt every step.  The inlet/outlet solver rebuilds every step.
+       Results follow the same neighbour rule either way. */
+    double verlet_skin;
 } esph_scene_desc;
 
 typedef struct esph_fluid_arrays {
@@ -300,14 +307,16 @@ int esph_scene_probe(esph_scene* scene, int32_t field, int64_t npoints, const do
    Timing is off unless enabled (it adds events to the stream). */
 int esph_scene_enable_timing(esph_scene* scene, int32_t enable);
 int esph_scene_phase_times(esph_scene* scene, double* ms5, int64_t* steps);
-/* out6: [0] particle-steps whose neighbour count exceeded the per-particle
+/* out8: [0] particle-steps whose candidate count exceeded the per-particle
    list width (those particles scanned their cells instead; results are the
    same), [1] fluid-fluid list width, [2] fluid-wall list width, [3] cells,
    [4] particles more than one cell outside the grid at the last sort (they
    share the border cells; many of them mean a blown-up simulation and slow
    steps), [5] error bits (1: non-finite position, 2: inlet copies dropped at
-   capacity). */
-int esph_scene_stats(esph_scene* scene, int64_t* out6);
+   capacity), [6] sorts so far (each rebuilds the candidate lists; the
+   upload's included), [7] exact-list width when candidate lists are reused
+   across steps (0 when they are rebuilt every step). */
+int esph_scene_stats(esph_scene* scene, int64_t* out8);
 /* Graph mode: the first step captures one step as a HIP graph; later steps
    replay it (the Shepard pass is then launched every step and skips itself
    on the device when not due).  Phase timing is not recorded in graph mode. */

@@ -316,3 +316,28 @@ def test_overflow_scan_path_matches_oracle(ctx):
     ref.step(20)
     assert dev.stats()["list_overflow"] > 0
     compare_fields(dev, ref, 1e-9)
+
+
+@pytest.mark.parametrize("skin", [-1.0, 0.0, 0.3])
+def test_candidate_list_reuse_keeps_parity(ctx, skin):
+    """Candidate lists reused across steps (skin 0.1 by default, 0.3) or
+    rebuilt every step (negative skin): the same pair sets and fields as the
+    oracle; with reuse, fewer sorts than steps."""
+    sc = scenarios.collapse_dry_2d(dx=0.04)
+    d = sc.desc(_lib.FP64)
+    d.verlet_skin = skin
+    dev = Scene(d, sc.fluid, sc.walls, ctx)
+    ref = O.from_scenario(sc)
+    for chunk in (37, 63):
+        dev.step(chunk)
+        ref.step(chunk)
+        dev.sync()
+        _pairs_equal(dev, ref)
+    compare_fields(dev, ref, 1e-9)
+    st = dev.stats()
+    say("skin", skin, st)
+    if skin < 0:
+        assert st["sorts"] == 101 and not st["reused_lists"]
+    else:
+        assert st["reused_lists"] and st["sorts"] < 101
+    dev.close()
