@@ -172,6 +172,7 @@ extern "C" int esph_scene_create(esph_ctx* ctx, const esph_scene_desc* desc, esp
     if (d.kernel == ESPH_WENDLAND_C2 && d.dim == 1) return fail(ctx, ESPH_ERR_ARG, "WendlandC2 has no 1D form");
     if (d.solver < ESPH_SOLVER_FLUID_WALL || d.solver > ESPH_SOLVER_FLUID_INLET_OUTLET_WALL)
         return fail(ctx, ESPH_ERR_ARG, "bad solver");
+    if (d.check_nplanes < 0 || d.check_nplanes > 8) return fail(ctx, ESPH_ERR_ARG, "check_nplanes must be 0..8");
     if (d.solver == ESPH_SOLVER_FLUID_INLET_OUTLET_WALL) {
         if (d.io_axis < 0 || d.io_axis >= d.dim) return fail(ctx, ESPH_ERR_ARG, "io_axis must be 0..dim-1");
         if (d.profile < ESPH_PROFILE_NONE || d.profile > ESPH_PROFILE_PIPE_3D ||
@@ -419,6 +420,10 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         hd.g[k] = k < D ? d.body_force[k] : 0.0;
         hd.check_min[k] = d.check_min[k];
         hd.check_max[k] = d.check_max[k];
+    }
+    hd.check_nplanes = d.check_enabled ? std::max(0, std::min(d.check_nplanes, 8)) : 0;
+    for (int q = 0; q < 8; ++q) {
+        for (int k = 0; k < 4; ++k) hd.check_planes[q][k] = d.check_planes[q][k];
     }
     build_grid(s, f, hd);
     const int nw = (int)s->walls.n;

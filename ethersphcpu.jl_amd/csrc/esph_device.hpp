@@ -102,6 +102,9 @@ struct SceneDev {
     double rho0, c0, p0, b, nu0, t0, cp, beta, dt;
     double g[3];
     double check_min[3], check_max[3];
+    double check_planes[8][4];  // deletion iff n . x > d for some plane q < check_nplanes
+    int32_t check_nplanes;
+    int32_t pad4;
     // cell grid: cells cs wide (cs >= R / stencil); neighbours of a cell lie
     // within `stencil` cells on each axis
     double lo[3];

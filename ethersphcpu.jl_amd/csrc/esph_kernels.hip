@@ -96,6 +96,12 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
         if (mode == 0 && check_due(s, dyn)) {
             for (int k = 0; k < D; ++k)
                 if (x[k] > s.check_max[k] || x[k] < s.check_min[k]) dead = true;
+            for (int q = 0; q < s.check_nplanes; ++q) {
+#pragma clang fp contract(off)
+                double dot = s.check_planes[q][0] * x[0] + s.check_planes[q][1] * x[1];
+                if (D == 3) dot = dot + s.check_planes[q][2] * x[2];
+                if (dot > s.check_planes[q][3]) dead = true;
+            }
         }
         // inlet/outlet migration (FluidInletOutletWallSolve.jl:196-262)
         int spawn_key = -1;

@@ -51,6 +51,7 @@ class SceneDesc(ctypes.Structure):
         ("outlet_exit", c_double), ("profile", c_int32), ("profile_terms", c_int32), ("outlet_profile", c_int32),
         ("profile_f", c_double), ("profile_nu", c_double), ("profile_l", c_double), ("profile_center", D3),
         ("verlet_skin", c_double),
+        ("check_nplanes", c_int32), ("check_planes", (c_double * 4) * 8),
     ]
 
 

@@ -37,11 +37,17 @@ PRECISIONS = {"FP64": _lib.FP64, "FP32_BULK": _lib.FP32_BULK}
 class AABBCheck:
     """The ``check`` predicate as data: a fluid particle is kept iff, on every
     axis, not (x > hi) and not (x < lo) -- the form of the reference examples'
-    closures (e.g. CollapseDry.jl:119-127).  Missing bounds are unbounded."""
+    closures (e.g. CollapseDry.jl:119-127) -- and, for every plane
+    (normal, d) in ``planes`` (at most 8), not (normal . x > d).  Missing
+    bounds are unbounded; together box and planes describe any convex
+    keep-region bounded by planes."""
 
-    def __init__(self, lo=None, hi=None):
+    def __init__(self, lo=None, hi=None, planes=()):
         self.lo = lo
         self.hi = hi
+        self.planes = [(tuple(float(c) for c in n), float(d)) for n, d in planes]
+        if len(self.planes) > 8:
+            raise ValueError("at most 8 check planes")
 
     def bounds(self, dim):
         lo = np.full(3, -np.inf)
@@ -141,6 +147,11 @@ def make_desc(dim, kernel: SmoothKernel_, model, td, solver: int, precision: int
         d.check_enabled = 1
         d.check_min = _lib.D3(*lo)
         d.check_max = _lib.D3(*hi)
+        d.check_nplanes = len(check.planes)
+        for q, (n, off) in enumerate(check.planes):
+            for k in range(3):
+                d.check_planes[q][k] = n[k] if k < len(n) else 0.0
+            d.check_planes[q][3] = off
     d.fluid_capacity = capacity
     if domain is not None:
         d.domain_set = 1

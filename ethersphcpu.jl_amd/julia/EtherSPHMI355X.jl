@@ -98,6 +98,8 @@ struct EsphSceneDesc
     profile_l::Float64
     profile_center::NTuple{3,Float64}
     verlet_skin::Float64
+    check_nplanes::Int32
+    check_planes::NTuple{32,Float64}   # 8 planes x (n_x, n_y, n_z, d)
 end
 
 struct EsphFluidArrays
@@ -174,7 +176,8 @@ function descriptor(dim, kernel, model, td, solver, check; planes = nothing, pro
                          td.dt_, td.density_reinitialized_interval_, td.check_interval_,
                          check === nothing ? 0 : 1, lo, hi, capacity, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0),
                          pl.axis - 1, pl.inlet_enter, pl.inlet_shift, pl.outlet_enter, pl.outlet_exit,
-                         pr.kind, pr.terms, pr.outlet_profile ? 1 : 0, pr.f, pr.nu, pr.l, pr.center, 0.0)
+                         pr.kind, pr.terms, pr.outlet_profile ? 1 : 0, pr.f, pr.nu, pr.l, pr.center, 0.0,
+                         0, ntuple(_ -> 0.0, 32))
 end
 
 fluid_arrays(n, x, v, rho, p, c, m, gap) =

@@ -194,6 +194,13 @@ typedef struct esph_scene_desc {
        rebuilt every step.  The inlet/outlet solver rebuilds every step.
        Results follow the same neighbour rule either way. */
     double verlet_skin;
+    /* further `check` predicates (used with check_enabled): a fluid particle
+       is also deleted at a check step iff for some q < check_nplanes (at
+       most 8): n_q . x > d_q, with check_planes[q] = (n_x, n_y, n_z, d_q) and
+       n_q . x evaluated as (n_x x + n_y y) + n_z z.  Together with the box
+       this expresses any convex keep-region bounded by planes. */
+    int32_t check_nplanes;
+    double check_planes[8][4];
 } esph_scene_desc;
 
 typedef struct esph_fluid_arrays {

@@ -971,6 +971,12 @@ static void step_once(eo_scene* s, int nthreads) {
             int out = 0;
             for (int k = 0; k < D; ++k)
                 if (s->x[i * D + k] > s->d.check_max[k] || s->x[i * D + k] < s->d.check_min[k]) out = 1;
+            for (int q = 0; q < s->d.check_nplanes && q < 8; ++q) {
+                const double* pl = s->d.check_planes[q];
+                double dot = pl[0] * s->x[i * D] + pl[1] * s->x[i * D + 1];
+                if (D == 3) dot = dot + pl[2] * s->x[i * D + 2];
+                if (dot > pl[3]) out = 1;
+            }
             if (out) continue;
             if (w != i) {
                 for (int k = 0; k < D; ++k) {

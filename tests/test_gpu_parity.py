@@ -246,6 +246,25 @@ def test_check_deletion_matches_oracle(ctx):
     _pairs_equal(dev, ref)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_check_planes_match_oracle(ctx, dim):
+    """Box plus tilted half-spaces: the same particles are deleted, stably."""
+    planes = [((0.6, 0.8, 0.0)[:dim], 0.11), ((-0.5, 0.3, 0.8)[:dim], 0.02)]
+    g = [3000.0, -2000.0] if dim == 2 else [3000.0, -2000.0, 1500.0]
+    sc = lattice_scene(n=16 if dim == 2 else 8, dx=0.01, dim=dim, check_interval=5,
+                       check=AABBCheck(lo=[-1.0] * dim, hi=[0.2] * dim, planes=planes), g=g, dt=1e-4)
+    say("check planes", dim)
+    dev = sc.make_scene(_lib.FP64, ctx)
+    ref = O.from_scenario(sc)
+    for _ in range(4):
+        dev.step(5)
+        ref.step(5)
+        assert dev.fluid_count() == ref.fluid_count()
+    assert ref.fluid_count() < sc.n_fluid
+    compare_fields(dev, ref, 1e-9)
+    _pairs_equal(dev, ref)
+
+
 def test_nan_position_is_reported(ctx):
     sc = lattice_scene(n=8, dx=0.01)
     sc.fluid.x_vec_[5, 0] = np.nan

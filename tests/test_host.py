@@ -59,6 +59,8 @@ int main(void) {
          offsetof(esph_scene_desc, check_max), offsetof(esph_scene_desc, domain_max),
          offsetof(esph_scene_desc, fluid_capacity));
   printf("%zu %zu\n", offsetof(esph_fluid_arrays, cp), offsetof(esph_wall_arrays, kappa));
+  printf("%zu %zu %zu %zu\n", offsetof(esph_scene_desc, io_axis), offsetof(esph_scene_desc, profile_center),
+         offsetof(esph_scene_desc, verlet_skin), offsetof(esph_scene_desc, check_planes));
   return 0;
 }
 '''
@@ -73,7 +75,9 @@ int main(void) {
     assert vals[:3] == [ctypes.sizeof(SD), ctypes.sizeof(FA), ctypes.sizeof(WA)]
     assert vals[3:8] == [SD.mu_0.offset, SD.dt.offset, SD.check_max.offset, SD.domain_max.offset,
                          SD.fluid_capacity.offset]
-    assert vals[8:] == [FA.cp.offset, WA.kappa.offset]
+    assert vals[8:10] == [FA.cp.offset, WA.kappa.offset]
+    assert vals[10:] == [SD.io_axis.offset, SD.profile_center.offset, SD.verlet_skin.offset,
+                         SD.check_planes.offset]
 
 
 def test_particle_constructors_zero_initialise():

@@ -260,3 +260,46 @@ def test_exact_cutoff_lattice_pairs():
     assert np.all(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] <= R * R)
     on_cut = np.isclose(np.hypot(d[:, 0], d[:, 1]), R, rtol=1e-12)
     assert 0 < on_cut.sum() < 9800  # some, not all, of the 3-dx lattice offsets are in
+
+
+def test_check_planes_delete_like_deleteat():
+    """A general check (box plus tilted half-spaces) on free particles:
+    the survivors after N steps are those a replay of check + deleteat! on
+    Python lists keeps, in order (FluidWallSolve.jl:89-97)."""
+    rng = np.random.default_rng(11)
+    n = 60
+    sc = lattice_scene(n=1, dx=0.001, check_interval=3,
+                       check=AABBCheck(lo=[-0.5, -0.5], hi=[0.5, 0.4],
+                                       planes=[((0.6, 0.8), 0.3), ((-1.0, 1.0), 0.35)]),
+                       dt=1e-3)
+    f = sc.fluid.subset(np.zeros(n, dtype=int))
+    # isolated particles: one per row 0.011 apart (rows farther apart than R),
+    # moving along x at constant speed
+    f.x_vec_ = np.column_stack([rng.uniform(-0.3, 0.3, n), np.arange(n) * 0.011 - 0.33])
+    f.v_vec_ = np.column_stack([rng.uniform(-40.0, 40.0, n), np.zeros(n)])
+    sc.fluid = f
+    R = sc.kernel.influence_radius_
+    assert R < 0.011
+    o = O.from_scenario(sc)
+    steps = 30
+    o.step(steps)
+    # replay
+    xs = [np.array(p, dtype=np.float64) for p in f.x_vec_]
+    vs = [np.array(v, dtype=np.float64) for v in f.v_vec_]
+    ids = list(range(n))
+    dt = sc.td.dt_
+    for step in range(1, steps + 1):
+        xs = [x + v * dt - 0.0 * (dt * dt) / 2.0 for x, v in zip(xs, vs)]
+        if step % 3 == 0:
+            keep = []
+            for k, x in enumerate(xs):
+                out = x[0] > 0.5 or x[1] > 0.4 or x[0] < -0.5 or x[1] < -0.5
+                out = out or (0.6 * x[0] + 0.8 * x[1]) > 0.3 or (-1.0 * x[0] + 1.0 * x[1]) > 0.35
+                if not out:
+                    keep.append(k)
+            xs = [xs[k] for k in keep]
+            vs = [vs[k] for k in keep]
+            ids = [ids[k] for k in keep]
+    assert 0 < len(ids) < n
+    assert np.array_equal(o.get(_lib.FIELD_ID).astype(int), ids)
+    np.testing.assert_array_equal(o.get(_lib.FIELD_X), np.array(xs))
