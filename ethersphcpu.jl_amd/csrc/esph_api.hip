@@ -446,7 +446,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     const size_t F = sizeof(float), G = sizeof(double), I = sizeof(int32_t);
     const size_t P = f32 ? F : G;
     size_t per_set = (size_t)cap * (2 * D * G + 5 * G + I + 8 * P) + (thermal ? (size_t)cap * (3 * G + P) : 0);
-    size_t bytes = 2 * per_set + (size_t)cap * (2 * D * G + G + (thermal ? G : 0) + 3 * I) +
+    size_t bytes = 2 * per_set + (size_t)cap * (G + 3 * I) +
                    (size_t)(hd.ncell + 1) * I * 2 + I + (size_t)hd.cell_tiles * I +
                    (size_t)nw * (3 * D * G + 3 * G + 2 * I + 9 * P) + (size_t)(hd.ncell + 1) * I +
                    sizeof(SceneDyn) + sizeof(SceneDev) +
@@ -459,10 +459,10 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     Arena& a = s->arena;
     bool ok = true;
     auto dal = [&](size_t n) { void* p = a.get(n); ok = ok && p; return p; };
+    // set[2 * parity + dpar]: arrays of particle order `parity` (changed by
+    // the sort) with per-step copy `dpar` (rewritten every step)
     for (int q = 0; q < 2; ++q) {
-        FluidSet& S = hd.set[q];
-        S.x = (double*)dal(G * cap * D);
-        S.v = (double*)dal(G * cap * D);
+        FluidSet S{};
         S.rho = (double*)dal(G * cap);
         S.p = (double*)dal(G * cap);
         S.c = (double*)dal(G * cap);
@@ -471,20 +471,31 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         S.id = (int32_t*)dal(I * cap);
         S.kind = io ? (int8_t*)dal(sizeof(int8_t) * cap) : nullptr;
         if (thermal) {
-            S.T = (double*)dal(G * cap);
             S.kappa = (double*)dal(G * cap);
             S.cp = (double*)dal(G * cap);
         }
-        const size_t P = f32 ? F : G;  // pair precision
-        S.A = dal(4 * P * cap);
-        S.B = dal(4 * P * cap);
         S.kap = (thermal && D == 3) ? dal(P * cap) : nullptr;
-        S.xf = dal(F * (D == 3 ? 4 : 2) * cap);
+        hd.set[2 * q] = S;
+        hd.set[2 * q + 1] = S;
     }
-    hd.xn = (double*)dal(G * cap * D);
-    hd.vn = (double*)dal(G * cap * D);
+    for (int q = 0; q < 2; ++q) {
+        double* x = (double*)dal(G * cap * D);
+        double* v = (double*)dal(G * cap * D);
+        double* T = thermal ? (double*)dal(G * cap) : nullptr;
+        void* A = dal(4 * P * cap);
+        void* B = dal(4 * P * cap);
+        void* xf = dal(F * (D == 3 ? 4 : 2) * cap);
+        for (int par = 0; par < 2; ++par) {
+            FluidSet& S = hd.set[2 * par + q];
+            S.x = x;
+            S.v = v;
+            S.T = T;
+            S.A = A;
+            S.B = B;
+            S.xf = xf;
+        }
+    }
     hd.rhon = (double*)dal(G * cap);
-    hd.Tn = thermal ? (double*)dal(G * cap) : nullptr;
     hd.key = (int32_t*)dal(I * cap);
     hd.slot = (int32_t*)dal(I * cap);
     hd.tmp = (int32_t*)dal(I * cap);
@@ -658,7 +669,11 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
     hipStream_t st = ctx->stream;
     HIPCHK(ctx, hipStreamSynchronize(st));
     const size_t G = sizeof(double);
+    // the upload starts at parity 0, per-step copy 0: the arrays of order 0
+    // take the constant fields; the positions, velocities and temperatures go
+    // to copy 1 (the step outputs' place) and the upload's sort gathers them
     FluidSet& S = hd.set[0];
+    const FluidSet& N = hd.set[1];
     // fluid, then inlet and outlet buffers; ids are sequence numbers per group
     std::vector<int32_t> ids(n);
     std::vector<int8_t> kinds(n, (int8_t)KIND_FLUID);
@@ -676,8 +691,8 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
         up(dst, a, G * na * w);
         if (nb) up(dst + na * w, b.data(), G * nb * w);
     };
-    up2(hd.xn, f->x, nf, s->bx, D);
-    up2(hd.vn, f->v, nf, s->bv, D);
+    up2(N.x, f->x, nf, s->bx, D);
+    up2(N.v, f->v, nf, s->bv, D);
     up2(hd.rhon, f->rho, nf, s->brho, 1);
     up2(S.p, f->p, nf, s->bp, 1);
     up2(S.c, f->c, nf, s->bc, 1);
@@ -686,7 +701,7 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
     up(S.id, ids.data(), sizeof(int32_t) * n);
     if (io) up(S.kind, kinds.data(), sizeof(int8_t) * n);
     if (thermal) {
-        up(hd.Tn, f->t, G * n);
+        up(N.T, f->t, G * n);
         up(S.kappa, f->kappa, G * n);
         up(S.cp, f->cp, G * n);
     }
@@ -818,7 +833,7 @@ static int group_order(esph_scene* s, int32_t group, SceneDyn& dyn, std::vector<
     if (rc) return rc;
     esph_ctx* ctx = s->ctx;
     const int n = dyn.n_live;
-    const FluidSet& S = s->h.set[dyn.parity];
+    const FluidSet& S = s->h.set[cur_set(dyn)];
     std::vector<int32_t> all(n);
     std::vector<int8_t> kinds;
     if (n) HIPCHK(ctx, hipMemcpy(all.data(), S.id, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
@@ -848,7 +863,7 @@ static int get_group(esph_scene* s, int32_t group, int32_t field, double* out, i
     int rc = group_order(s, group, dyn, order, ids);
     if (rc) return rc;
     const int n = (int)order.size();
-    const FluidSet& S = s->h.set[dyn.parity];
+    const FluidSet& S = s->h.set[cur_set(dyn)];
     const double* src = nullptr;
     int w = 1;
     switch (field) {

@@ -35,7 +35,8 @@ constexpr double kDefaultSkin = 0.05;
 // captured step graph can be replayed without the host.
 struct SceneDyn {
     int64_t step;    // completed steps (the reference's `step` of the last one)
-    int32_t parity;  // which FluidSet is current
+    int32_t parity;  // particle order (static arrays) of the current state: 0 or 1
+    int32_t dpar;    // which copy of the per-step arrays holds the current state: 0 or 1
     int32_t n_live;  // fluid particles in the current set
     int32_t err;     // bit 0: a non-finite position was met
     int32_t overflow;  // particle-steps whose neighbour count exceeded the list width
@@ -49,15 +50,29 @@ struct SceneDyn {
     int32_t sorted;    // this step's decision: sort (and rebuild) or commit in place
     uint32_t disp2_bits;  // float bits of this step's largest squared displacement from xref
     int32_t nsorts;       // sorts (list rebuilds) so far, the upload's included
+    int32_t pad;
 };
 
-// One sorted copy of the fluid state.  Masters are binary64 SoA arrays (state
-// integration and downloads).  A, B (and kap) are the packed records the
-// pair loops gather, in the pair precision T (float in FP32_BULK, double in
-// FP64), 4 values per particle so one record is one or two vector loads:
+// Index into SceneDev::set of the current state and of the step's outputs
+// (same particle order, the other per-step copy).
+__host__ __device__ inline int cur_set(const SceneDyn& d) { return 2 * d.parity + d.dpar; }
+__host__ __device__ inline int next_set(const SceneDyn& d) { return 2 * d.parity + (d.dpar ^ 1); }
+
+// A view of the fluid state in one particle order.  Masters are binary64
+// SoA arrays (state integration and downloads).  A, B (and kap) are the
+// packed records the pair loops gather, in the pair precision T (float in
+// FP32_BULK, double in FP64), 4 values per particle so one record is one or
+// two vector loads:
 //   3D: A = (vx, vy, vz, m)   B = (rho, p/rho^2, gap, T)   kap = kappa
 //   2D: A = (vx, vy, m, gap)  B = (rho, p/rho^2, T, kappa)
-// The sort writes A, B.zw and kap; the continuity pass writes B.xy.
+// The arrays come in two kinds.  Per-step arrays (x, v, T, A, B, xf) are
+// rewritten by every step: the force pass writes the new state into the
+// other copy (`dpar ^ 1`), and a step without a sort just switches copies.
+// The other arrays change only through the sort (which permutes every array
+// into the other order) or in place (rho, p, c by the continuity pass).
+// SceneDev::set[2 * parity + dpar] pairs the arrays of order `parity` with
+// per-step copy `dpar`.  The sort writes A, B.zw and kap; the continuity pass
+// writes B.xy; the force pass writes whole A and B records.
 struct FluidSet {
     double* x;      // n*dim
     double* v;      // n*dim
@@ -114,11 +129,8 @@ struct SceneDev {
     double cs;
     int32_t stencil;
     int32_t pad1;
-    // fluid
-    FluidSet set[2];
-    double* xn;      // step outputs in the current set's order
-    double* vn;
-    double* Tn;
+    // fluid: set[2 * parity + dpar] (see FluidSet)
+    FluidSet set[4];
     double* rhon;    // Shepard output / upload staging
     int32_t* key;
     int32_t* slot;

@@ -87,8 +87,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
         const int D = s.dim;
         double x[3] = {0.0, 0.0, 0.0};
         bool finite = true;
+        const double* xnew = s.set[next_set(dyn)].x;  // the step's (or the upload's) positions
         for (int k = 0; k < D; ++k) {
-            x[k] = s.xn[i * D + k];
+            x[k] = xnew[i * D + k];
             finite = finite && isfinite(x[k]);
         }
         int key;
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
         // inlet/outlet migration (FluidInletOutletWallSolve.jl:196-262)
         int spawn_key = -1;
         if (mode == 0 && (s.flags & FLAG_IO)) {
-            const FluidSet& S = s.set[dyn.parity];
+            const FluidSet& S = s.set[cur_set(dyn)];
             const int kd = S.kind[i], a = s.io_axis;
             if (kd == KIND_OUTLET && x[a] > s.outlet_exit) dead = true;
             if (kd == KIND_FLUID && x[a] > s.outlet_enter) s.leave_id[atomicAdd(&s.dyn->nleave, 1)] = S.id[i];
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
             const int e = atomicAdd(&s.dyn->nspawn, 1);
             if (dyn.n_live + e < s.cap) {
                 s.spawn_src[e] = i;
-                s.spawn_id[e] = s.set[dyn.parity].id[i];
+                s.spawn_id[e] = s.set[cur_set(dyn)].id[i];
                 s.spawn_key[e] = spawn_key;
                 s.spawn_slot[e] = atomicAdd(&s.count[spawn_key], 1);
             } else {
@@ -317,14 +318,18 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
         int rank = 0;
         for (int w = b; w < e; ++w) rank += s.tmp[w] < q ? 1 : 0;
         const int p = b + rank;
-        const FluidSet& S = s.set[dyn.parity];
-        const FluidSet& O = s.set[dyn.parity ^ 1];
+        // from the step's outputs (N) and the current order's other arrays
+        // (S) into the other order, per-step copy dpar (free once the step's
+        // outputs exist)
+        const FluidSet& S = s.set[cur_set(dyn)];
+        const FluidSet& N = s.set[next_set(dyn)];
+        const FluidSet& O = s.set[2 * (dyn.parity ^ 1) + dyn.dpar];
         const int D = s.dim;
         const bool thermal = (s.flags & FLAG_THERMAL) != 0;
         double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < D; ++k) {
-            x[k] = s.xn[src * D + k];
-            v[k] = s.vn[src * D + k];
+            x[k] = N.x[src * D + k];
+            v[k] = N.v[src * D + k];
         }
         int8_t kd = S.kind ? S.kind[src] : (int8_t)KIND_FLUID;
         const bool src_fluid = kd == KIND_FLUID;
@@ -373,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
         if (O.kind) O.kind[p] = kd;
         double Tt = 0.0, kap = 0.0;
         if (thermal) {
-            Tt = s.Tn[src];
+            Tt = N.T[src];
             kap = S.kappa[src];
             O.T[p] = Tt;
             O.kappa[p] = kap;
@@ -414,16 +419,9 @@ __global__ void k_sort_decide(const SceneDev* __restrict__ scenes, int nscenes, 
     d->sorted = (mode != 0 || !s.verlet || (s.flags & FLAG_IO) || check_due(s, *d) || moved) ? 1 : 0;
 }
 
-// A step without sort: the step's outputs replace the current set in place
-// (same order and cells, so the candidate lists stay valid).
-template <typename T>
-__device__ __forceinline__ void commit_records(const SceneDev& s, const FluidSet& S, int D, int i, const double* v,
-                                               double Tt, bool thermal) {
-    const double kap = thermal ? S.kappa[i] : 0.0;
-    // float records hold T - t0 (see k_sort_scatter)
-    store_sorted_rec<T>(S, D, i, v, S.m[i], S.gap[i], sizeof(T) == 4 ? Tt - s.t0 : Tt, kap, thermal);
-}
-
+// A step without sort keeps the particle order: the step's outputs are
+// already in the other per-step copy (k_sort_finish switches to it).  Only a
+// Shepard step's densities move here, into the order's rho array.
 __global__ __launch_bounds__(kBlock) void k_commit(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, int ntiles) {
     for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
@@ -431,34 +429,10 @@ __global__ __launch_bounds__(kBlock) void k_commit(const SceneDev* __restrict__ 
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
-        if (dyn.sorted) continue;
+        if (dyn.sorted || !shepard_due(s, dyn)) continue;
         const int i = tile.begin + (int)threadIdx.x;
         if (i >= dyn.n_live) continue;
-        const FluidSet& S = s.set[dyn.parity];
-        const int D = s.dim;
-        double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
-        for (int k = 0; k < D; ++k) {
-            x[k] = s.xn[i * D + k];
-            v[k] = s.vn[i * D + k];
-            S.x[i * D + k] = x[k];
-            S.v[i * D + k] = v[k];
-        }
-        if (D == 3)
-            static_cast<float4*>(S.xf)[i] = make_float4((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]),
-                                                        (float)(x[2] - s.lo[2]), 0.0f);
-        else
-            static_cast<float2*>(S.xf)[i] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
-        if (shepard_due(s, dyn)) S.rho[i] = s.rhon[i];
-        const bool thermal = (s.flags & FLAG_THERMAL) != 0;
-        double Tt = 0.0;
-        if (thermal) {
-            Tt = s.Tn[i];
-            S.T[i] = Tt;
-        }
-        if (s.prec == ESPH_FP32_BULK)
-            commit_records<float>(s, S, D, i, v, Tt, thermal);
-        else
-            commit_records<double>(s, S, D, i, v, Tt, thermal);
+        s.set[cur_set(dyn)].rho[i] = s.rhon[i];
     }
 }
 
@@ -474,10 +448,11 @@ __global__ void k_sort_finish(const SceneDev* __restrict__ scenes, int nscenes, 
             d->next_outlet_id += d->nleave;
         }
         d->n_live = s.cell_start[s.ncell];
-        d->parity ^= 1;
+        d->parity ^= 1;  // the sort wrote the other order, per-step copy dpar
         d->rebuild = 1;
         d->nsorts += 1;
     } else {
+        d->dpar ^= 1;  // the step's outputs become the current state
         d->rebuild = 0;
     }
     d->novf = 0;
@@ -504,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const SceneDev* __restrict__ s
         const SceneDyn dyn = *s.dyn;
         const int i = tile.begin + (int)threadIdx.x;
         if (i >= dyn.n_live) continue;
-        const FluidSet& S = s.set[dyn.parity];
+        const FluidSet& S = s.set[cur_set(dyn)];
         const int8_t* kind = S.kind;  // inlet/outlet solver: fluid particles only
         int64_t w = offs ? offs[i] : 0;
         int cnt = 0;
@@ -549,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_probe(const SceneDev* __restrict__ s
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
     const SceneDyn dyn = *s.dyn;
-    const FluidSet& S = s.set[dyn.parity];
+    const FluidSet& S = s.set[cur_set(dyn)];
     const double* f = S.rho;
     int w = 1;
     if (field == ESPH_FIELD_V) { f = S.v; w = D; }

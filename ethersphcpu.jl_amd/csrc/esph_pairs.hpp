@@ -24,7 +24,7 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
         for_work<SCAN>(s, dyn, tile, [&](int i) {
-        const FluidSet& S = s.set[dyn.parity];
+        const FluidSet& S = s.set[cur_set(dyn)];
         if (S.kind && S.kind[i] != KIND_FLUID) {
             // inlet/outlet buffers keep rho and p (drho is reset,
             // FluidInletOutletWallSolve.jl:91-99) and need no lists
@@ -123,29 +123,48 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
 //   conduction 2 k_eq r W' (T_i - T_j) / rho_i / rho_j / (r^2 + .01 h^2) m_j / cp_i
 //              = 4 k_i k_j r W' (T_i - T_j) m_j / ((k_i + k_j) rho_j (r^2 + .01 h^2)) / (rho_i cp_i)
 // and 1 / W(gap) is taken per particle when both gaps are equal.
-// New x, v, T go to xn, vn, Tn in the current particle order.
+// The new state goes to the other per-step copy N in the current particle
+// order: x, v, T, the float position and whole A and B records (B.xy keep
+// this step's density and pressure until the next continuity pass).
 // ---------------------------------------------------------------------------
-// Buffer particles of the inlet/outlet solver: dv = 0, then updatePosition!
-// (FluidInletOutletWallSolve.jl:170-186): x += v dt - 0 dt^2 / 2.
 template <int D>
-__device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S, int i) {
+__device__ __forceinline__ void store_xf(const SceneDev& s, const FluidSet& N, int i, const double* x) {
+    if (D == 3)
+        static_cast<float4*>(N.xf)[i] = make_float4((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]),
+                                                    (float)(x[2] - s.lo[2]), 0.0f);
+    else
+        static_cast<float2*>(N.xf)[i] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
+}
+
+// Buffer particles of the inlet/outlet solver: dv = 0, then updatePosition!
+// (FluidInletOutletWallSolve.jl:170-186): x += v dt - 0 dt^2 / 2; their
+// records and temperature carry over unchanged.
+template <int D, typename T>
+__device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S, const FluidSet& N, int i) {
 #pragma clang fp contract(off)
+    using V = typename Vec4<T>::type;
     const double dt = s.dt;
+    double x[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         const double v = S.v[i * D + k];
-        s.vn[i * D + k] = v;
-        s.xn[i * D + k] = S.x[i * D + k] + (v * dt - 0.0 * (dt * dt) / 2.0);
+        N.v[i * D + k] = v;
+        x[k] = S.x[i * D + k] + (v * dt - 0.0 * (dt * dt) / 2.0);
+        N.x[i * D + k] = x[k];
     }
+    store_xf<D>(s, N, i, x);
+    static_cast<V*>(N.A)[i] = static_cast<const V*>(S.A)[i];
+    static_cast<V*>(N.B)[i] = static_cast<const V*>(S.B)[i];
+    if (N.T) N.T[i] = S.T[i];
 }
 
 // Returns the particle's squared displacement from its position when the
 // candidate lists were built (0 when the lists are rebuilt every step).
 template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH>
-__device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet& S, int i) {
+__device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet& S, const FluidSet& N, int i) {
     using KQ = KernelQ<T, K>;
     if (S.kind && S.kind[i] != KIND_FLUID) {
-        buffer_move<D>(s, S, i);
+        buffer_move<D, T>(s, S, N, i);
         return 0.0;
     }
     const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
@@ -278,9 +297,10 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
 #pragma clang fp contract(off)
         const double dt = s.dt;
         double bf[3];
+        double Tn = 0.0;
         if (THERMAL) {
-            const double Tn = S.T[i] + dt * (double)dT;
-            s.Tn[i] = Tn;
+            Tn = S.T[i] + dt * (double)dT;
+            N.T[i] = Tn;
 #pragma unroll
             for (int k = 0; k < D; ++k) bf[k] = -s.g[k] * s.beta * (Tn - s.t0);
         } else {
@@ -288,20 +308,45 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
             for (int k = 0; k < D; ++k) bf[k] = s.g[k];
         }
         double disp2 = 0.0;
+        double xnew[3] = {0.0, 0.0, 0.0}, vnew[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             double v = S.v[i * D + k];
             v += bf[k] * dt;
             const double dvk = (double)dv[k];
             v += dvk * dt;
-            s.vn[i * D + k] = v;
+            N.v[i * D + k] = v;
+            vnew[k] = v;
             const double xk = xi[k] + (v * dt - dvk * (dt * dt) / 2.0);
-            s.xn[i * D + k] = xk;
+            N.x[i * D + k] = xk;
+            xnew[k] = xk;
             if (s.verlet) {
                 const double e = xk - s.xref[i * D + k];
                 disp2 += e * e;
             }
         }
+        store_xf<D>(s, N, i, xnew);
+        // records of the new state; float records hold T - t0 (see k_sort_scatter)
+        using V = typename Vec4<T>::type;
+        const T Tr = THERMAL ? (T)(sizeof(T) == 4 ? Tn - s.t0 : Tn) : me.Tt;
+        V a, b;
+        a.x = (T)vnew[0];
+        a.y = (T)vnew[1];
+        b.x = rho_i;
+        b.y = pr2_i;
+        if (D == 3) {
+            a.z = (T)vnew[2];
+            a.w = m_i;
+            b.z = gap_i;
+            b.w = Tr;
+        } else {
+            a.z = m_i;
+            a.w = gap_i;
+            b.z = Tr;
+            b.w = kap_i;
+        }
+        static_cast<V*>(N.A)[i] = a;
+        static_cast<V*>(N.B)[i] = b;
         return disp2;
     }
 }
@@ -318,7 +363,8 @@ __global__ __launch_bounds__(kBlock) void k_forces(const SceneDev* __restrict__ 
         double dmax = 0.0;
         for_work<SCAN>(s, dyn, tile, [&](int i) {
             if (!SCAN && overflowed(s, i)) return;
-            const double d2 = forces_body<D, K, T, THERMAL, SCAN, DEPTH>(s, s.set[dyn.parity], i);
+            const double d2 =
+                forces_body<D, K, T, THERMAL, SCAN, DEPTH>(s, s.set[cur_set(dyn)], s.set[next_set(dyn)], i);
             dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
         });
         if (s.verlet) {
@@ -357,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__
         if (s.kreinit <= 0 || (dyn.step + 1) % s.kreinit != 0) continue;
         for_work<SCAN>(s, dyn, tile, [&](int i) {
         if (!SCAN && overflowed(s, i)) return;
-        const FluidSet& S = s.set[dyn.parity];
+        const FluidSet& S = s.set[cur_set(dyn)];
         const int8_t* kind = S.kind;
         if (kind && kind[i] != KIND_FLUID) return;
         double xi[3];
