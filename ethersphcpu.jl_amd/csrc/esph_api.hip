@@ -548,6 +548,8 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     }
     ctx->used += a.total;
     hipStream_t st = ctx->stream;
+    // the sort expects zero cell counts (its scan clears them after use)
+    HIPCHK(ctx, hipMemsetAsync(hd.count, 0, I * (hd.ncell + 1), st));
     // wall binning and upload
     {
         const WallHost& w = s->walls;

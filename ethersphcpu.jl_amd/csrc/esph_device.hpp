@@ -50,7 +50,7 @@ struct SceneDyn {
     int32_t sorted;    // this step's decision: sort (and rebuild) or commit in place
     uint32_t disp2_bits;  // float bits of this step's largest squared displacement from xref
     int32_t nsorts;       // sorts (list rebuilds) so far, the upload's included
-    int32_t pad;
+    int32_t scan_done;    // cell-count tiles scanned in this sort (reset by the last one)
 };
 
 // Index into SceneDev::set of the current state and of the step's outputs

@@ -19,23 +19,9 @@ namespace esph {
 // (FluidWallSolve.jl:89-97, deleteat! is stable) sends deleted particles to
 // the tail bucket `ncell`, which is dropped.
 // mode 0: end of a step; mode 1: upload (rho from rhon, no check).
+// The cell counts are zero on entry: the allocation clears them and the
+// scan's last phase clears them again once read.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_sort_clear(const SceneDev* __restrict__ scenes,
-                                                       const Tile* __restrict__ ctiles, int ntiles) {
-    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
-        const int t = tile_index(tb, ntiles);
-        const Tile tile = ctiles[t];
-        const SceneDev& s = scenes[tile.scene];
-        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
-#pragma unroll
-        for (int q = 0; q < kScanItems; ++q) {
-            const int c = tile.begin + q * kBlock + (int)threadIdx.x;
-            if (c <= s.ncell) s.count[c] = 0;
-        }
-        if (tile.begin == 0 && threadIdx.x == 0) s.dyn->outside = 0;
-    }
-}
-
 __device__ __forceinline__ int cell_key(const SceneDev& s, int D, const double* x) {
     const int cx = cell_coord(x[0], s.lo[0], s.inv_cs, s.nc[0]);
     const int cy = cell_coord(x[1], s.lo[1], s.inv_cs, s.nc[1]);
@@ -80,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
         const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
-        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
+        if (!s.dyn->sorted) continue;  // a step without sort
         const SceneDyn dyn = *s.dyn;
         const int i = tile.begin + (int)threadIdx.x;
         if (i >= dyn.n_live) continue;
@@ -174,14 +160,31 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
     return off + incl - v;
 }
 
-// Phase 1: scan of kCellTile counts per tile; tile total to bsum.
+// Exclusive scan of a scene's tile totals (bsum, in place) by one block.
+__device__ __forceinline__ void scan_tile_totals(const SceneDev& s) {
+    int carry = 0;
+    for (int base = 0; base < s.cell_tiles; base += kBlock) {
+        const int q = base + (int)threadIdx.x;
+        const int v = q < s.cell_tiles ? s.bsum[q] : 0;
+        int total;
+        const int ex = block_exclusive_scan(v, &total);
+        if (q < s.cell_tiles) s.bsum[q] = carry + ex;
+        carry += total;
+    }
+}
+
+// Phases 1 and 2: scan of kCellTile counts per tile, tile total to bsum; the
+// last of a scene's tiles to finish (a ticket counter, no waiting) then scans
+// the scene's tile totals.  Release: each block's total is fenced before its
+// ticket; acquire: the last block fences after its ticket, before reading.
 __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ ctiles, int ntiles) {
+    __shared__ int last;
     for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
         const int t = tile_index(tb, ntiles);
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
-        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
+        if (!s.dyn->sorted) continue;  // a step without sort
         const int c0 = tile.begin + (int)threadIdx.x * kScanItems;
         int v[kScanItems];
         int sum = 0;
@@ -197,35 +200,30 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
             if (c0 + q <= s.ncell) s.cell_start[c0 + q] = run;
             run += v[q];
         }
-        if (threadIdx.x == 0) s.bsum[tile.begin / kCellTile] = total;
-    }
-}
-
-// Phase 2: one block per scene scans its tile totals (exclusive, in place).
-__global__ __launch_bounds__(kBlock) void k_sort_scan2(const SceneDev* __restrict__ scenes, int nscenes) {
-    for (int sc = blockIdx.x; sc < nscenes; sc += gridDim.x) {
-        const SceneDev& s = scenes[sc];
-        if (!s.dyn->sorted) continue;
-        int carry = 0;
-        for (int base = 0; base < s.cell_tiles; base += kBlock) {
-            const int q = base + (int)threadIdx.x;
-            const int v = q < s.cell_tiles ? s.bsum[q] : 0;
-            int total;
-            const int ex = block_exclusive_scan(v, &total);
-            if (q < s.cell_tiles) s.bsum[q] = carry + ex;
-            carry += total;
+        if (threadIdx.x == 0) {
+            s.bsum[tile.begin / kCellTile] = total;
+            __threadfence();
+            last = atomicAdd(&s.dyn->scan_done, 1) == s.cell_tiles - 1;
         }
+        __syncthreads();
+        if (last) {
+            __threadfence();
+            scan_tile_totals(s);
+            if (threadIdx.x == 0) s.dyn->scan_done = 0;
+        }
+        __syncthreads();
     }
 }
 
 // Phase 3: add tile offsets; the last cell also writes the grand total.
+// The counts are read for the last time here and cleared.
 __global__ __launch_bounds__(kBlock) void k_sort_scan3(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ ctiles, int ntiles) {
     for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
         const int t = tile_index(tb, ntiles);
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
-        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
+        if (!s.dyn->sorted) continue;  // a step without sort
         const int off = s.bsum[tile.begin / kCellTile];
 #pragma unroll
         for (int q = 0; q < kScanItems; ++q) {
@@ -234,6 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan3(const SceneDev* __restric
                 const int st = s.cell_start[c] + off;
                 s.cell_start[c] = st;
                 if (c == s.ncell) s.cell_start[c + 1] = st + s.count[c];
+                s.count[c] = 0;  // ready for the next sort
             }
         }
     }
@@ -245,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_place(const SceneDev* __restric
         const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
-        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
+        if (!s.dyn->sorted) continue;  // a step without sort
         const int i = tile.begin + (int)threadIdx.x;
         const SceneDyn dyn = *s.dyn;
         if (i < dyn.n_live) {
@@ -296,8 +295,15 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
         const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
-        if (!s.dyn->sorted) continue;  // a step without sort: k_commit
         const SceneDyn dyn = *s.dyn;
+        if (!dyn.sorted) {
+            // a step without sort keeps the particle order: its outputs are
+            // already in the other per-step copy (k_sort_finish switches to
+            // it); only a Shepard step's densities move, into the order's rho
+            const int i = tile.begin + (int)threadIdx.x;
+            if (shepard_due(s, dyn) && i < dyn.n_live) s.set[cur_set(dyn)].rho[i] = s.rhon[i];
+            continue;
+        }
         const bool io = mode == 0 && (s.flags & FLAG_IO);
         const int ns = io ? spawn_count(s, dyn) : 0;
         const int q = tile.begin + (int)threadIdx.x;
@@ -416,24 +422,9 @@ __global__ void k_sort_decide(const SceneDev* __restrict__ scenes, int nscenes, 
     SceneDyn* d = s.dyn;
     const float disp2 = __uint_as_float(d->disp2_bits);
     const bool moved = !((double)disp2 <= s.disp2_limit);
-    d->sorted = (mode != 0 || !s.verlet || (s.flags & FLAG_IO) || check_due(s, *d) || moved) ? 1 : 0;
-}
-
-// A step without sort keeps the particle order: the step's outputs are
-// already in the other per-step copy (k_sort_finish switches to it).  Only a
-// Shepard step's densities move here, into the order's rho array.
-__global__ __launch_bounds__(kBlock) void k_commit(const SceneDev* __restrict__ scenes,
-                                                   const Tile* __restrict__ tiles, int ntiles) {
-    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
-        const int t = tile_index(tb, ntiles);
-        const Tile tile = tiles[t];
-        const SceneDev& s = scenes[tile.scene];
-        const SceneDyn dyn = *s.dyn;
-        if (dyn.sorted || !shepard_due(s, dyn)) continue;
-        const int i = tile.begin + (int)threadIdx.x;
-        if (i >= dyn.n_live) continue;
-        s.set[cur_set(dyn)].rho[i] = s.rhon[i];
-    }
+    const int sorted = (mode != 0 || !s.verlet || (s.flags & FLAG_IO) || check_due(s, *d) || moved) ? 1 : 0;
+    d->sorted = sorted;
+    if (sorted) d->outside = 0;  // counted again by this sort
 }
 
 __global__ void k_sort_finish(const SceneDev* __restrict__ scenes, int nscenes, int mode) {
@@ -576,14 +567,11 @@ hipError_t launch_shepard(const Bucket& b, hipStream_t st) {
 hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st) {
     dim3 blk(kBlock);
     hipLaunchKernelGGL(k_sort_decide, dim3((p.nscenes + 63) / 64), dim3(64), 0, st, p.scenes, p.nscenes, mode);
-    hipLaunchKernelGGL(k_sort_clear, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
     hipLaunchKernelGGL(k_sort_bin, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_scan1, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
-    hipLaunchKernelGGL(k_sort_scan2, dim3(grid_for(p.nscenes)), blk, 0, st, p.scenes, p.nscenes);
     hipLaunchKernelGGL(k_sort_scan3, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
     hipLaunchKernelGGL(k_sort_place, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_scatter, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
-    hipLaunchKernelGGL(k_commit, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles);
     hipLaunchKernelGGL(k_sort_finish, dim3((p.nscenes + 63) / 64), dim3(64), 0, st, p.scenes, p.nscenes, mode);
     return hipGetLastError();
 }

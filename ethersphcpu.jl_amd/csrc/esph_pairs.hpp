@@ -160,8 +160,13 @@ __device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S
 
 // Returns the particle's squared displacement from its position when the
 // candidate lists were built (0 when the lists are rebuilt every step).
+// full_records: the next copy's records were last written in another
+// particle order (the first step after a sort), so B is written whole;
+// otherwise B.zw there still hold this order's gap (and, without heat, T) and
+// B.xy is rewritten by the next continuity pass before any read.
 template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH>
-__device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet& S, const FluidSet& N, int i) {
+__device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet& S, const FluidSet& N, int i,
+                                              bool full_records) {
     using KQ = KernelQ<T, K>;
     if (S.kind && S.kind[i] != KIND_FLUID) {
         buffer_move<D, T>(s, S, N, i);
@@ -346,7 +351,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
             b.w = kap_i;
         }
         static_cast<V*>(N.A)[i] = a;
-        static_cast<V*>(N.B)[i] = b;
+        if (THERMAL || full_records) static_cast<V*>(N.B)[i] = b;
         return disp2;
     }
 }
@@ -364,7 +369,8 @@ __global__ __launch_bounds__(kBlock) void k_forces(const SceneDev* __restrict__ 
         for_work<SCAN>(s, dyn, tile, [&](int i) {
             if (!SCAN && overflowed(s, i)) return;
             const double d2 =
-                forces_body<D, K, T, THERMAL, SCAN, DEPTH>(s, s.set[cur_set(dyn)], s.set[next_set(dyn)], i);
+                forces_body<D, K, T, THERMAL, SCAN, DEPTH>(s, s.set[cur_set(dyn)], s.set[next_set(dyn)], i,
+                                                           dyn.rebuild != 0);
             dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
         });
         if (s.verlet) {
