@@ -451,7 +451,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
                    (size_t)nw * (3 * D * G + 3 * G + 2 * I + 9 * P) + (size_t)(hd.ncell + 1) * I +
                    sizeof(SceneDyn) + sizeof(SceneDev) +
                    ((size_t)(cap + kBlock - 1) / kBlock + (size_t)hd.cell_tiles) * sizeof(Tile) +
-                   (size_t)cap * I * (hd.kff * (hd.verlet ? 2 : 1) + hd.kfw + 4) + 2 * (size_t)cap * 4 * F +
+                   (size_t)cap * I * (hd.kff + hd.kfw + 4) + 2 * (size_t)cap * 4 * F +
                    (size_t)cap * D * G;
     if (ctx->budget && ctx->used + bytes > ctx->budget)
         return fail(ctx, ESPH_ERR_BUDGET, "scene needs " + std::to_string(bytes) + " bytes; budget left " +
@@ -504,8 +504,6 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.bsum = (int32_t*)dal(I * hd.cell_tiles);
     hd.vnbr = (int32_t*)dal(I * (size_t)cap * hd.kff);
     hd.nvnb = (int32_t*)dal(I * (size_t)cap);
-    hd.nbr = hd.verlet ? (int32_t*)dal(I * (size_t)cap * hd.kff) : hd.vnbr;
-    hd.nnb = hd.verlet ? (int32_t*)dal(I * (size_t)cap) : hd.nvnb;
     hd.xref = (double*)dal(G * (size_t)cap * D);
     hd.wnbr = (int32_t*)dal(I * (size_t)cap * std::max(hd.kfw, 1));
     hd.nnbw = (int32_t*)dal(I * (size_t)cap);

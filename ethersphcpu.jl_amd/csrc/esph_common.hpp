@@ -325,25 +325,6 @@ __device__ __forceinline__ bool overflowed(const SceneDev& s, int i) {
     return s.nvnb[i] > s.kff || (s.nw > 0 && s.nnbw[i] > s.kfw);
 }
 
-// Appends j to particle i's chunked list through a 4-entry register buffer
-// (call list_flush once after the last entry).
-struct ListWriter {
-    int4 buf = make_int4(0, 0, 0, 0);
-    int cnt = 0;
-    __device__ __forceinline__ void add(int32_t* __restrict__ list, size_t cap, int i, int j) {
-        const int r = cnt & 3;
-        buf.x = r == 0 ? j : buf.x;
-        buf.y = r == 1 ? j : buf.y;
-        buf.z = r == 2 ? j : buf.z;
-        buf.w = r == 3 ? j : buf.w;
-        if (r == 3) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
-        ++cnt;
-    }
-    __device__ __forceinline__ void flush(int32_t* __restrict__ list, size_t cap, int i) {
-        if ((cnt & 3) != 0) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
-    }
-};
-
 // Velocity record of a neighbour (continuity pass).
 template <typename T>
 struct ARec {

@@ -27,9 +27,10 @@ enum : int {
 enum : int8_t { KIND_FLUID = 0, KIND_INLET = 1, KIND_OUTLET = 2 };
 
 // Candidate-list skin (fraction of R) when the descriptor leaves it at 0.
-// Swept on the 1M-particle 3D dam break: 0.03-0.06 within 1% of each other
-// (0.1: 9% slower, the lists carry 15% more candidates).
-constexpr double kDefaultSkin = 0.05;
+// Every pair pass walks the candidates, so their excess (1 + skin)^D - 1
+// costs on every step and a rebuild on every skin R / 2 of motion: swept on
+// the 1M-particle 3D dam break, 0.03 is best (0.02 and 0.04 within 1%).
+constexpr double kDefaultSkin = 0.03;
 
 // Per-scene state that changes every step and lives in device memory, so a
 // captured step graph can be replayed without the host.
@@ -142,10 +143,9 @@ struct SceneDev {
     int32_t kff;          // neighbour-list width (fluid-fluid)
     int32_t kfw;          // neighbour-list width (fluid-wall)
     int32_t pad0;
-    // per-step neighbour lists in 4-entry chunks: entry k of particle i at
-    // [((k / 4) * cap + i) * 4 + k % 4]
-    int32_t* nbr;         // this step's fluid-fluid neighbours (exact rule), when verlet
-    int32_t* nnb;         // their count per particle
+    // neighbour lists in 4-entry chunks: entry k of particle i at
+    // [((k / 4) * cap + i) * 4 + k % 4]; the pair passes apply the exact rule
+    // to the candidates every step
     int32_t* vnbr;        // fluid-fluid candidates within Rv (the neighbours themselves when !verlet)
     int32_t* nvnb;        // their count per particle (overflow when > kff)
     int32_t* wnbr;        // fluid-wall candidates within Rv (filtered by the exact rule when used)

@@ -11,8 +11,12 @@ namespace esph {
 //   drho_i = sum_j m_j (v_i - v_j) . gradW_ij,  gradW_ij = W'(r) r_ij / r
 // then updateDensity! (SelfAction/UpdateDensity.jl:8-12) and updatePressure!
 // (SelfAction/UpdatePressure.jl:8-16).  Writes rho, p, c and the record's
-// (rho, p/rho^2).  This pass also records the step's neighbour lists
-// (positions of the step start), which the force and Shepard passes reuse.
+// (rho, p/rho^2).  Right after a sort this pass also builds the candidate
+// lists (every particle within Rv = R (1 + skin)), which all pair passes walk
+// until the next sort, applying the exact rule d^2 <= R^2 to each candidate.
+// Candidates beyond R are weighted by zero rather than branched around: the
+// sums equal those over the neighbours alone, and the wavefront stays
+// convergent.
 // ---------------------------------------------------------------------------
 template <int D, int K, typename T, bool SCAN, int DEPTH>
 __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restrict__ scenes,
@@ -28,7 +32,6 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
         if (S.kind && S.kind[i] != KIND_FLUID) {
             // inlet/outlet buffers keep rho and p (drho is reset,
             // FluidInletOutletWallSolve.jl:91-99) and need no lists
-            s.nnb[i] = 0;
             s.nvnb[i] = 0;
             if (s.nw > 0) s.nnbw[i] = 0;
             return;
@@ -57,11 +60,9 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
                 return;
             }
         }
-        // with reused candidate lists the step's neighbours (exact rule) are
-        // written to nbr for the force and Shepard passes
-        const bool exact_list = !SCAN && s.verlet;
+        // candidates beyond R (the lists hold every particle within Rv)
+        // contribute exactly zero: weighted by 0 instead of branched around
         const double R2 = s.R2;
-        ListWriter lw;
         T drho = T(0);
         auto load = [&](int j) {
             ARec<T> a;
@@ -69,9 +70,8 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
             return a;
         };
         for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
-                                    [&](int j, double d2, const double* rv, const ARec<T>& a) {
-            if (!(d2 <= R2)) return;
-            if (exact_list) lw.add(s.nbr, (size_t)s.cap, i, j);
+                                    [&](int, double d2, const double* rv, const ARec<T>& a) {
+            const T in = d2 <= R2 ? T(1) : T(0);
             const T* vj = a.v;
             const T m_j = a.m;
             const T r = psqrt((T)d2);
@@ -82,12 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
             T dot = T(0);
 #pragma unroll
             for (int k = 0; k < D; ++k) dot += (vi[k] - vj[k]) * (T)rv[k];
-            drho += m_j * sg * dot;
+            drho += m_j * sg * dot * in;
         });
-        if (exact_list) {
-            lw.flush(s.nbr, (size_t)s.cap, i);
-            s.nnb[i] = lw.cnt;
-        }
         {
 #pragma clang fp contract(off)
             double rho = S.rho[i];
@@ -107,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Momentum (and heat) pass + integration, pull form over the step's lists.
+// Momentum (and heat) pass + integration, pull form over the candidate lists.
 //  FF: pressureForce! (PressureForce.jl:8-21), viscosityForce!
 //      (ViscosityForce.jl:8-25), thermalConduction! (ThermalConduction.jl:8-22)
 //  FW: wallForce! (WallForce.jl:8-30), pressureForce! wall
@@ -190,8 +186,11 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         load_rec<D, T, THERMAL>(S, j, o);
         return o;
     };
-    for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.nbr, s.nnb[i], load,
+    // candidates beyond R contribute exactly zero (weighted, not branched)
+    const double R2 = s.R2;
+    for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
                                 [&](int, double d2, const double* rv, const Rec<T>& o) {
+        const T in = d2 <= R2 ? T(1) : T(0);
         const T r = psqrt((T)d2);
         T W, dW;
         KQ::wdw(r * inv_h, kA, kB, W, dW);
@@ -207,12 +206,12 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         P += fabs(P) * T(0.01) * W * inv_Wg;
         // viscosity
         const T r2e = r * r + eta2;
-        const T vf = T(4) * nu0 * r * dW * prcp((rho_i + o.rho) * r2e);
-        const T ps = -P * sg;
+        const T vf = T(4) * nu0 * r * dW * prcp((rho_i + o.rho) * r2e) * in;
+        const T ps = -P * sg * in;
 #pragma unroll
         for (int k = 0; k < D; ++k) dv[k] += o.m * (ps * (T)rv[k] + vf * (me.v[k] - o.v[k]));
         if (THERMAL)
-            dT += T(4) * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e);
+            dT += T(4) * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e) * in;
     });
     if (THERMAL) dT *= prcp(rho_i * (T)S.cp[i]);
     if (s.nw > 0) {
@@ -357,7 +356,10 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
 }
 
 template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH>
-__global__ __launch_bounds__(kBlock) void k_forces(const SceneDev* __restrict__ scenes,
+// Float forces without heat are held to 4 waves per SIMD (128 VGPRs): the
+// gather pipeline otherwise takes a few registers more and falls to 3.
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value && !THERMAL && !SCAN ? 4 : 1))) void k_forces(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, int ntiles) {
     for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
         const int t = tile_index(tb, ntiles);
@@ -426,8 +428,10 @@ __global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__
             r.kind = kind ? kind[j] : (int8_t)KIND_FLUID;
             return r;
         };
-        for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.nbr, s.nnb[i], load,
+        const double R2 = s.R2;
+        for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
                                     [&](int, double d2, const double*, const SRec& o) {
+            if (!(d2 <= R2)) return;           // candidates within Rv: the exact rule
             if (o.kind != KIND_FLUID) return;  // fluid-fluid pairs only
             double W, dW;
             KQ::wdw(sqrt(d2) * s.inv_h, s.kA, s.kB, W, dW);
