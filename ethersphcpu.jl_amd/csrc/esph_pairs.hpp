@@ -431,12 +431,12 @@ __global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__
         const double R2 = s.R2;
         for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
                                     [&](int, double d2, const double*, const SRec& o) {
-            if (!(d2 <= R2)) return;           // candidates within Rv: the exact rule
-            if (o.kind != KIND_FLUID) return;  // fluid-fluid pairs only
+            // weight 1 for neighbours under the exact rule among fluid particles
+            const double in = d2 <= R2 && o.kind == KIND_FLUID ? 1.0 : 0.0;
             double W, dW;
             KQ::wdw(sqrt(d2) * s.inv_h, s.kA, s.kB, W, dW);
             const double rho_j = o.rho;
-            const double a = W * o.m / rho_j;
+            const double a = W * o.m / rho_j * in;
             sw += a;
             ss += a * rho_j;
         });
