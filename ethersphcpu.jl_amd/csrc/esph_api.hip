@@ -329,6 +329,26 @@ static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) 
     hd.cs = cs;
     hd.stencil = (int32_t)std::ceil(hd.Rv / cs);
     if ((double)hd.stencil * cs <= hd.Rv) hd.stencil += 1;
+    // position quantum (FluidSet::xq): the smallest power of two with the
+    // grid extent within 2^30 quanta
+    double extent = 0.0;
+    for (int k = 0; k < D; ++k) extent = std::max(extent, (double)hd.nc[k] * cs);
+    const int e = (int)std::ceil(std::log2(std::max(extent, 1e-300) / 1073741824.0));
+    hd.qu = std::ldexp(1.0, e);
+    hd.inv_qu = std::ldexp(1.0, -e);
+    hd.quf = (float)hd.qu;
+    // band for the quantized float distance near R (d <= 1.01 R): each axis
+    // of rv is off the binary64 difference by at most one quantum (two
+    // half-quantum roundings) plus one float rounding of the integer
+    // difference, dx = u + 2^-24 |rv|; then |d2 - d^2| <= 2 sqrt(D) d dx
+    // + D dx^2, the float sum adds D + 1 roundings of d^2 and R2f one of R^2.
+    // Doubled for margin.
+    const double eps = std::ldexp(1.0, -24), dd = 1.01 * hd.R;
+    const double dx = hd.qu + eps * dd;
+    const double band = 2.0 * std::sqrt((double)D) * dd * dx + D * dx * dx + (D + 1) * eps * dd * dd +
+                        eps * hd.R2;
+    hd.qbandf = (float)(2.0 * band);
+    hd.R2f = (float)hd.R2;
 }
 
 static int cell_of_host(const SceneDev& hd, int D, const double* x) {
@@ -451,7 +471,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
                    (size_t)nw * (3 * D * G + 3 * G + 2 * I + 9 * P) + (size_t)(hd.ncell + 1) * I +
                    sizeof(SceneDyn) + sizeof(SceneDev) +
                    ((size_t)(cap + kBlock - 1) / kBlock + (size_t)hd.cell_tiles) * sizeof(Tile) +
-                   (size_t)cap * I * (hd.kff + hd.kfw + 4) + 2 * (size_t)cap * 4 * F +
+                   (size_t)cap * I * (hd.kff + hd.kfw + 4) + 2 * (size_t)cap * 4 * F + (f32 ? 2 * (size_t)cap * 4 * I : 0) +
                    (size_t)cap * D * G;
     if (ctx->budget && ctx->used + bytes > ctx->budget)
         return fail(ctx, ESPH_ERR_BUDGET, "scene needs " + std::to_string(bytes) + " bytes; budget left " +
@@ -485,6 +505,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         void* A = dal(4 * P * cap);
         void* B = dal(4 * P * cap);
         void* xf = dal(F * (D == 3 ? 4 : 2) * cap);
+        void* xq = f32 ? dal(I * (D == 3 ? 4 : 2) * cap) : nullptr;
         for (int par = 0; par < 2; ++par) {
             FluidSet& S = hd.set[2 * par + q];
             S.x = x;
@@ -493,6 +514,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
             S.A = A;
             S.B = B;
             S.xf = xf;
+            S.xq = xq;
         }
     }
     hd.rhon = (double*)dal(G * cap);

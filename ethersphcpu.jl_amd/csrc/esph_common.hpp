@@ -306,6 +306,103 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
     }
 }
 
+// Quantized positions (FluidSet::xq, FP32_BULK).
+template <int D>
+__device__ __forceinline__ void store_xq(const SceneDev& s, const FluidSet& N, int i, const double* x) {
+    int32_t q[3];
+    quantize(s.lo, s.inv_qu, D, x, q);
+    if (D == 3)
+        static_cast<int4*>(N.xq)[i] = make_int4(q[0], q[1], q[2], 0);
+    else
+        static_cast<int2*>(N.xq)[i] = make_int2(q[0], q[1]);
+}
+
+template <int D>
+__device__ __forceinline__ void load_q(const void* xq, int j, int32_t* q) {
+    if (D == 3) {
+        const int4 v = static_cast<const int4*>(xq)[j];
+        q[0] = v.x;
+        q[1] = v.y;
+        q[2] = v.z;
+    } else {
+        const int2 v = static_cast<const int2*>(xq)[j];
+        q[0] = v.x;
+        q[1] = v.y;
+        q[2] = 0;
+    }
+}
+
+// Pair geometry from quantized positions: rv = (q_i - q_j) qu (the integer
+// difference is exact; the float conversion and the power-of-two quantum
+// round at most once), d2 = |rv|^2 in float.  Against the FP64 rule
+// d^2 <= R^2 on the binary64 positions, |d2 - d^2| stays below qbandf
+// near R (two half-quanta per axis, the float roundings, R2f's own
+// rounding; derived where qbandf is set), so a candidate outside the band
+// is decided by the float distance and one inside it -- or with a position
+// beyond the quantized range -- by the FP64 rule itself.  Returns whether j
+// is a neighbour; rv and d2 are the float geometry of the pair.
+template <int D>
+__device__ __forceinline__ bool qgeom(const SceneDev& s, const double* __restrict__ xall, const double* xi,
+                                      const int32_t* qi, const int32_t* qj, int j, float* rv, float& d2) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) rv[k] = (float)(int32_t)((uint32_t)qi[k] - (uint32_t)qj[k]) * s.quf;
+    if (D == 2) rv[2] = 0.0f;
+    d2 = D == 3 ? rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2] : rv[0] * rv[0] + rv[1] * rv[1];
+    bool in = d2 <= s.R2f;
+    if (qi[0] == kQInvalid || qj[0] == kQInvalid || fabsf(d2 - s.R2f) <= s.qbandf) {
+        double xj[3], rvd[3];
+        load_x<D>(xall, j, xj);
+        const double d2d = dist2_exact<D>(xi, xj, rvd);
+        in = d2d <= s.R2;
+#pragma unroll
+        for (int k = 0; k < D; ++k) rv[k] = (float)rvd[k];
+        d2 = (float)d2d;
+    }
+    return in;
+}
+
+// The list walk of for_pairs_d with the candidates' geometry from their
+// quantized positions (one 16-byte or 8-byte load instead of the binary64
+// position): body(j, in, d2, rv, record) with in = the FP64 rule's verdict.
+template <int D, int DEPTH, typename Load, typename Body>
+__device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S, const double* xi, int i,
+                                            const int32_t* __restrict__ list, int cnt, Load&& load, Body&& body) {
+    static_assert(DEPTH == 1 || DEPTH == 2 || DEPTH == 4, "DEPTH divides a 4-entry chunk");
+    if (cnt <= 0) return;
+    int32_t qi[3];
+    load_q<D>(S.xq, i, qi);
+    const size_t cap = (size_t)s.cap;
+    const int nq = (cnt + 3) >> 2;
+    int4 cur = *reinterpret_cast<const int4*>(list + list_at(cap, 0, i));
+    for (int q = 0; q < nq; ++q) {
+        int4 nxt = cur;
+        if (q + 1 < nq) nxt = *reinterpret_cast<const int4*>(list + list_at(cap, 4 * (q + 1), i));
+        const int js[4] = {cur.x, cur.y, cur.z, cur.w};
+        const int m = cnt - 4 * q;  // entries of this chunk: min(4, m)
+#pragma unroll
+        for (int g = 0; g < 4; g += DEPTH) {
+            if (g < m) {
+                int32_t qj[DEPTH][3];
+                decltype(load(0)) rec[DEPTH];
+#pragma unroll
+                for (int u = 0; u < DEPTH; ++u) {
+                    load_q<D>(S.xq, js[g + u], qj[u]);
+                    rec[u] = load(js[g + u]);
+                }
+#pragma unroll
+                for (int u = 0; u < DEPTH; ++u) {
+                    if (g + u < m) {
+                        float rv[3], d2;
+                        const bool in = qgeom<D>(s, S.x, xi, qi, qj[u], js[g + u], rv, d2);
+                        body(js[g + u], in, d2, rv, rec[u]);
+                    }
+                }
+            }
+        }
+        cur = nxt;
+    }
+}
+
 // The SCAN instances walk the step's overflow list with a small fixed grid:
 // kOvfTiles tiles of kBlock entries per scene, striding over the list.
 // Calls f(i) for the particle(s) of this thread: the tile slot (main

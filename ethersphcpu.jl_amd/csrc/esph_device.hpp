@@ -94,6 +94,12 @@ struct FluidSet {
     // 2D): the neighbour-list build tests candidates on these and falls back
     // to the FP64 rule only inside a rounding band around R
     void* xf;
+    // FP32_BULK only (null otherwise): positions relative to the grid origin
+    // in whole quanta SceneDev::qu (int4 in 3D with w = 0, int2 in 2D; every
+    // component kQInvalid when a coordinate is beyond 2^30 quanta or not
+    // finite).  Differences of these are exact, so the continuity pass takes
+    // its pair geometry from one 16-byte load per candidate (see qgeom).
+    void* xq;
 };
 
 struct SceneDev {
@@ -130,6 +136,11 @@ struct SceneDev {
     double cs;
     int32_t stencil;
     int32_t pad1;
+    // position quantum of FluidSet::xq: a power of two with the grid extent
+    // below 2^30 quanta; qband: half-width of the band around R^2 inside which
+    // the quantized float distance cannot decide the FP64 rule (see qgeom)
+    double qu, inv_qu;
+    float quf, qbandf, R2f, pad5;
     // fluid: set[2 * parity + dpar] (see FluidSet)
     FluidSet set[4];
     double* rhon;    // Shepard output / upload staging
@@ -239,6 +250,20 @@ __device__ __forceinline__ double dist2_exact(const double* a, const double* b, 
         rv[2] = d2;
     }
     return s;
+}
+
+// Quantized position of x (see FluidSet::xq); also used by the host.
+constexpr int32_t kQInvalid = INT32_MIN;
+__host__ __device__ inline void quantize(const double* lo, double inv_qu, int D, const double* x, int32_t* q) {
+    bool ok = true;
+    for (int k = 0; k < D; ++k) {
+        const double t = (x[k] - lo[k]) * inv_qu;  // inv_qu is a power of two: exact scaling
+        ok = ok && t > -1073741824.0 && t < 1073741824.0;  // |t| < 2^30 (false for NaN)
+        q[k] = ok ? (int32_t)rint(t) : 0;
+    }
+    for (int k = D; k < 3; ++k) q[k] = 0;
+    if (!ok)
+        for (int k = 0; k < 3; ++k) q[k] = kQInvalid;
 }
 
 // ---------------------------------------------------------------------------

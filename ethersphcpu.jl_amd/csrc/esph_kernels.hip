@@ -370,6 +370,14 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
                                                         (float)(x[2] - s.lo[2]), 0.0f);
         else
             static_cast<float2*>(O.xf)[p] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
+        if (O.xq) {
+            int32_t qx[3];
+            quantize(s.lo, s.inv_qu, D, x, qx);
+            if (D == 3)
+                static_cast<int4*>(O.xq)[p] = make_int4(qx[0], qx[1], qx[2], 0);
+            else
+                static_cast<int2*>(O.xq)[p] = make_int2(qx[0], qx[1]);
+        }
         for (int k = 0; k < D; ++k) s.xref[p * D + k] = x[k];  // the candidate lists are rebuilt from here
         const bool rho_new = mode == 1 || (shepard_due(s, dyn) && src_fluid && !spawn);
         const double rho = rho_new ? s.rhon[src] : S.rho[src];

@@ -69,21 +69,33 @@ __global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restric
             load_A<D, T>(S.A, j, a.v, a.m, a.gap);
             return a;
         };
-        for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
-                                    [&](int, double d2, const double* rv, const ARec<T>& a) {
-            const T in = d2 <= R2 ? T(1) : T(0);
+        // one candidate: in = 1 for a neighbour under the exact rule, else 0
+        auto pair = [&](T in, T d2, const T* rv, const ARec<T>& a) {
             const T* vj = a.v;
             const T m_j = a.m;
-            const T r = psqrt((T)d2);
+            const T r = psqrt(d2);
             T W, dW;
             KQ::wdw(r * inv_h, kA, kB, W, dW);
             // normalize(0) is NaN and the reference zeroes it (Neighbour.jl:54-59)
-            const T sg = d2 == 0.0 ? T(0) : dW * prcp(r);
+            const T sg = d2 == T(0) ? T(0) : dW * prcp(r);
             T dot = T(0);
 #pragma unroll
-            for (int k = 0; k < D; ++k) dot += (vi[k] - vj[k]) * (T)rv[k];
+            for (int k = 0; k < D; ++k) dot += (vi[k] - vj[k]) * rv[k];
             drho += m_j * sg * dot * in;
-        });
+        };
+        if constexpr (sizeof(T) == 4 && !SCAN) {
+            // FP32_BULK: geometry from the quantized positions (FP64 rule near R)
+            for_pairs_q<D, DEPTH>(s, S, xi, i, s.vnbr, cnt, load,
+                                  [&](int, bool in, float d2, const float* rv, const ARec<T>& a) {
+                pair(in ? T(1) : T(0), d2, rv, a);
+            });
+        } else {
+            for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
+                                        [&](int, double d2, const double* rv, const ARec<T>& a) {
+                T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
+                pair(d2 <= R2 ? T(1) : T(0), (T)d2, rvt, a);
+            });
+        }
         {
 #pragma clang fp contract(off)
             double rho = S.rho[i];
@@ -130,6 +142,7 @@ __device__ __forceinline__ void store_xf(const SceneDev& s, const FluidSet& N, i
                                                     (float)(x[2] - s.lo[2]), 0.0f);
     else
         static_cast<float2*>(N.xf)[i] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
+    if (N.xq) store_xq<D>(s, N, i, x);
 }
 
 // Buffer particles of the inlet/outlet solver: dv = 0, then updatePosition!
@@ -236,7 +249,8 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         const double R2 = s.R2;
         for_pairs_d<D, SCAN, DEPTH>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], wload,
                                     [&](int j, double d2, const double* rv, const WRec& w) {
-            if (!(d2 <= R2)) return;  // candidates within Rv: the exact rule
+            // candidates within Rv: the exact rule selects the terms (no branch)
+            const bool in = d2 <= R2;
             const V wa = w.wa;
             const V wb = w.wb;
             T n[3], vw[3], gap_j, Tw, kw;
@@ -273,7 +287,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 const T a = r * r - psi * psi;
                 const T xi_ = psqrt(a > T(0) ? a : T(0));
                 const T q = psi / gap_j;
-                if (!(q > T(1) || xi_ > gap_j)) {
+                if (in && !(q > T(1) || xi_ > gap_j)) {
                     const T p_xi = fabs((T(1) + cos(T(3.14159265358979323846) * xi_ / gap_j)) * T(0.5));
                     const T beta = vn > T(0) ? T(0) : T(1);
                     const T wf = (T(0.01) * (c_i * c_i) + beta * c_i * fabs(vn)) * inv_h * fabs(T(1) - q) *
@@ -283,16 +297,18 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 }
             }
             const T r2e = r * r + eta2;
-            const T vf = T(2) * nu0 * r * dW * inv_rho_i * prcp(r2e);
+            T vf = T(2) * nu0 * r * dW * inv_rho_i * prcp(r2e);
             T ps = T(0);
             if (wall_pressure) {
                 T P = pr2_i + P0;
                 P += fabs(P) * T(0.01) * W * inv_Wgap_i;
                 ps = -P * sg;
             }
+            vf = in ? vf : T(0);
+            ps = in ? ps : T(0);
 #pragma unroll
             for (int k = 0; k < D; ++k) dv[k] += m_i * (ps * (T)rv[k] + vf * (vv[k] - vn * n[k]));
-            if (THERMAL && s.wkind[j] == ESPH_WALL_THERMOSTATIC)
+            if (THERMAL && in && s.wkind[j] == ESPH_WALL_THERMOSTATIC)
                 dTw += (kap_i + kw) * r * dW * (T_i - Tw) * prcp(r2e);
         });
         if (THERMAL) dT += dTw * th_w;
