@@ -30,18 +30,14 @@ for p in (os.path.join(ROOT, "ethersphcpu.jl_amd"), ROOT):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-# Algorithmic bytes of one launch of the force pass (forces + integration),
-# 3D.  Per fluid particle: reads x, v (binary64), its A and B records, c and
-# the list-build position xref; writes x, v, the float position and the A
-# record of the next state.  Per wall particle: its position and records.
-#   FP32_BULK: 24+24+16+16+8+24 + 24+24+16+16 = 192 B; wall 24+16+16 = 56 B
-#   FP64:      24+24+32+32+8+24 + 24+24+16+32 = 240 B; wall 24+32+32 = 88 B
-FORCE_BYTES = {0: (240.0, 88.0), 1: (192.0, 56.0)}
-# Whole step per fluid / wall particle-step, 3D: continuity (reads x, A, rho;
-# writes rho, p, c, B.xy: FP32_BULK 80 B, FP64 104 B) + the force pass +
-# Shepard every 30 steps (x, m, rho read, rhon written: 48 B / 30); the sort
-# (a few percent of the steps) is left out.
-STEP_BYTES = {0: (240.0 + 104.0 + 1.6, 88.0), 1: (192.0 + 80.0 + 1.6, 56.0)}
+# Algorithmic bytes per fluid / wall particle of one launch of the force pass
+# (forces + integration), 3D, by the survey's counting rule (every
+# per-particle array the phase must touch, read once and written once;
+# neighbour re-reads are cache traffic): FP64 form 128 B / 56 B, FP32_BULK
+# 160 B / 28 B.
+FORCE_BYTES = {0: (128.0, 56.0), 1: (160.0, 28.0)}
+# Whole-step algorithmic bytes per fluid / wall particle-step, 3D, Shepard every 30.
+STEP_BYTES = {0: (241.6, 56.0), 1: (232.8, 28.0)}
 HBM_PEAK_GBS = 8000.0
 
 
