@@ -55,18 +55,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_gpu(sc, prec, steps, warmup):
+def time_gpu(sc, prec, steps, warmup, phase_steps=200):
+    """Steps of one scene replayed from its captured step graph (the timed
+    region: `steps` steps with the state resident in HBM), then `phase_steps`
+    more with plain launches and per-phase HIP events for the breakdown."""
     from ethersph import _lib
 
     dev = sc.make_scene(prec)
+    dev.enable_graph(True)
     dev.step(warmup)
     dev.sync()
-    dev.enable_timing(True)
-    dev.phase_times()  # reset
     t0 = time.perf_counter()
     dev.step(steps)
     dev.sync()
     wall = time.perf_counter() - t0
+    dev.enable_graph(False)
+    dev.enable_timing(True)
+    dev.phase_times()  # reset
+    dev.step(phase_steps)
+    dev.sync()
     ph = dev.phase_times()
     ph["stats"] = dev.stats()
     tuned = dev.ctx.tuned_depth(sc.dim, sc.kernel.kind, prec, 1)
