@@ -110,6 +110,28 @@ extern "C" int esph_ctx_set_tuning(esph_ctx* ctx, int32_t mode, int32_t depth) {
     ctx->fixed_depth = mode == 0 ? depth : 2;
     ctx->depth.clear();
     ctx->depth_ms.clear();
+    ctx->block.clear();
+    ctx->block_ms.clear();
+    return ESPH_OK;
+}
+
+extern "C" int esph_ctx_set_workgroup(esph_ctx* ctx, int32_t threads) {
+    if (!ctx || (threads != 64 && threads != 128 && threads != 256))
+        return fail(ctx, ESPH_ERR_ARG, "set_workgroup: 64, 128 or 256 threads");
+    ctx->fixed_block = threads;
+    return ESPH_OK;
+}
+
+extern "C" int esph_ctx_tuned_workgroup(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision,
+                                        int32_t thermal_mix, int32_t* threads, float* ms3) {
+    if (!ctx || !threads) return ESPH_ERR_ARG;
+    auto it = ctx->block.find({dim, kernel, precision, thermal_mix});
+    if (it == ctx->block.end()) return ESPH_ERR_STATE;
+    *threads = it->second;
+    if (ms3) {
+        const auto& m = ctx->block_ms.at(it->first);
+        for (int q = 0; q < 3; ++q) ms3[q] = m[q];
+    }
     return ESPH_OK;
 }
 

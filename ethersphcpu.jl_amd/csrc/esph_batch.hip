@@ -30,45 +30,68 @@ std::array<int, 4> depth_key(const Bucket& b) {
 void apply_depth(const esph_ctx* ctx, Bucket& b) {
     if (!ctx->tune) {
         b.depth = ctx->fixed_depth;
+        b.bs = ctx->fixed_block;
         return;
     }
     auto it = ctx->depth.find(depth_key(b));
     b.depth = it == ctx->depth.end() ? 0 : it->second;
+    auto ib = ctx->block.find(depth_key(b));
+    b.bs = ib == ctx->block.end() ? kBlock : ib->second;
+}
+
+// Shortest of three timed force passes of b (after one untimed), in ms.
+static hipError_t time_forces(Bucket& b, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float& t) {
+    hipError_t e = launch_forces(b, st);  // warm-up
+    t = 1e30f;
+    for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+        e = hipEventRecord(e0, st);
+        if (e == hipSuccess) e = launch_forces(b, st);
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        t = std::min(t, ms);
+    }
+    return e;
 }
 
 hipError_t tune_depth(esph_ctx* ctx, Bucket& b, hipStream_t st) {
-    const int cand[3] = {1, 2, 4};
-    std::array<float, 3> best{};
+    const int dcand[3] = {1, 2, 4};
+    const int bcand[3] = {64, 128, kBlock};
+    std::array<float, 3> dbest{}, bbest{};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
+    // gather depth at kBlock-thread workgroups, then the workgroup size at that depth
+    b.bs = kBlock;
     for (int c = 0; c < 3 && e == hipSuccess; ++c) {
-        b.depth = cand[c];
-        e = launch_forces(b, st);  // warm-up
-        float t = 1e30f;
-        for (int r = 0; r < 3 && e == hipSuccess; ++r) {
-            e = hipEventRecord(e0, st);
-            if (e == hipSuccess) e = launch_forces(b, st);
-            if (e == hipSuccess) e = hipEventRecord(e1, st);
-            if (e == hipSuccess) e = hipEventSynchronize(e1);
-            float ms = 0.f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-            t = std::min(t, ms);
-        }
-        best[c] = t;
+        b.depth = dcand[c];
+        e = time_forces(b, st, e0, e1, dbest[c]);
+    }
+    int k = 0;
+    for (int c = 1; c < 3; ++c)
+        if (dbest[c] < dbest[k]) k = c;
+    b.depth = dcand[k];
+    bbest[2] = dbest[k];
+    for (int c = 0; c < 2 && e == hipSuccess; ++c) {
+        b.bs = bcand[c];
+        e = time_forces(b, st, e0, e1, bbest[c]);
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (e != hipSuccess) {
         b.depth = 2;
+        b.bs = kBlock;
         return e;
     }
-    int k = 0;
-    for (int c = 1; c < 3; ++c)
-        if (best[c] < best[k]) k = c;
-    b.depth = cand[k];
+    int kb = 2;
+    for (int c = 0; c < 2; ++c)
+        if (bbest[c] < bbest[kb]) kb = c;
+    b.bs = bcand[kb];
     ctx->depth[depth_key(b)] = b.depth;
-    ctx->depth_ms[depth_key(b)] = best;
+    ctx->depth_ms[depth_key(b)] = dbest;
+    ctx->block[depth_key(b)] = b.bs;
+    ctx->block_ms[depth_key(b)] = bbest;
     return hipSuccess;
 }
 
@@ -149,7 +172,7 @@ hipError_t StepPlan::launch_step(hipStream_t st) const {
 int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
     hipStream_t st = ctx->stream;
     // buckets of an untuned class: the first step runs eagerly and tunes
-    // their gather depth between its continuity and force passes
+    // their gather depth and workgroup size between its continuity and force passes
     bool untuned = false;
     for (Bucket& b : buckets) {
         apply_depth(ctx, b);
@@ -172,9 +195,13 @@ int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
         if (e != hipSuccess) return hip_fail(ctx, "tuning step", e);
         nsteps -= 1;
     }
-    // a graph captured with other depths (tuning settings changed) is recaptured
+    // a graph captured with other depths or workgroup sizes (tuning settings
+    // changed) is recaptured
     std::vector<int> depths;
-    for (const Bucket& b : buckets) depths.push_back(b.depth);
+    for (const Bucket& b : buckets) {
+        depths.push_back(b.depth);
+        depths.push_back(b.bs);
+    }
     if (exec && depths != captured_depth) {
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);

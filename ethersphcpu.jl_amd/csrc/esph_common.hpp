@@ -409,8 +409,9 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
 // kernels) or, in the SCAN instances, every kOvfTiles*kBlock-th entry of the
 // step's overflow list.
 template <bool SCAN, typename F>
-__device__ __forceinline__ void for_work(const SceneDev& s, const SceneDyn& dyn, const Tile& tile, F&& f) {
-    const int q0 = tile.begin + (int)threadIdx.x;
+__device__ __forceinline__ void for_work(const SceneDev& s, const SceneDyn& dyn, const Tile& tile, int first,
+                                         F&& f) {
+    const int q0 = (SCAN ? tile.begin : first) + (int)threadIdx.x;
     if (SCAN) {
         for (int q = q0; q < dyn.novf; q += kOvfTiles * kBlock) f(s.ovf[q]);
     } else if (q0 < dyn.n_live) {
@@ -516,9 +517,18 @@ static inline int grid_for(int ntiles) { return ntiles > 0 ? ntiles : 1; }
 template <typename L>
 static hipError_t launch_both(const Bucket& b, hipStream_t st, L&& launch) {
     if (b.ntiles <= 0) return hipSuccess;
-    launch(std::false_type{}, dim3(grid_for(b.ntiles)), b.tiles, b.ntiles);
-    launch(std::true_type{}, dim3(kOvfTiles * b.nscenes), b.ovf_tiles, kOvfTiles * b.nscenes);
+    launch(std::false_type{}, b.tiles, b.ntiles);
+    launch(std::true_type{}, b.ovf_tiles, kOvfTiles * b.nscenes);
     return hipGetLastError();
 }
+
+// Workgroup size of the pair passes (threads), chosen per bucket by the tuner.
+#define ESPH_DISPATCH_BS(BVAR, BODY)                         \
+    switch (BVAR) {                                          \
+        case 64: { constexpr int BSZ = 64; BODY; } break;    \
+        case 128: { constexpr int BSZ = 128; BODY; } break;  \
+        default: { constexpr int BSZ = kBlock; BODY; } break; \
+    }
+#define SINGLE_ARG(...) __VA_ARGS__
 
 }  // namespace esph

@@ -21,6 +21,7 @@ struct Bucket {
     bool has_thermal;        // some scene of the bucket is thermal
     bool has_plain;          // some scene of the bucket is not
     int depth = 2;           // gather depth of the pair passes (1, 2 or 4; see the tuner)
+    int bs = 256;            // workgroup size of the pair passes (64, 128 or 256; see the tuner)
 };
 
 // Tables for the dimension-agnostic sort kernels of a whole batch.

@@ -18,16 +18,19 @@ namespace esph {
 // sums equal those over the neighbours alone, and the wavefront stays
 // convergent.
 // ---------------------------------------------------------------------------
-template <int D, int K, typename T, bool SCAN, int DEPTH>
-__global__ __launch_bounds__(kBlock) void k_continuity(const SceneDev* __restrict__ scenes,
+template <int D, int K, typename T, bool SCAN, int DEPTH, int BS>
+__global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles, int ntiles) {
     using KQ = KernelQ<T, K>;
-    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
-        const int t = tile_index(tb, ntiles);
-        const Tile tile = tiles[t];
+    constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
+    const int nv = ntiles * SUB;
+    for (int vb = blockIdx.x; vb < nv; vb += gridDim.x) {
+        const int v = tile_index(vb, nv);
+        const Tile tile = tiles[v / SUB];
+        const int first = tile.begin + (v % SUB) * BS;
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
-        for_work<SCAN>(s, dyn, tile, [&](int i) {
+        for_work<SCAN>(s, dyn, tile, first, [&](int i) {
         const FluidSet& S = s.set[cur_set(dyn)];
         if (S.kind && S.kind[i] != KIND_FLUID) {
             // inlet/outlet buffers keep rho and p (drho is reset,
@@ -371,20 +374,23 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     }
 }
 
-template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH>
 // Float forces without heat are held to 4 waves per SIMD (128 VGPRs): the
 // gather pipeline otherwise takes a few registers more and falls to 3.
-__global__ __launch_bounds__(kBlock)
+template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH, int BS>
+__global__ __launch_bounds__(SCAN ? kBlock : BS)
 __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value && !THERMAL && !SCAN ? 4 : 1))) void k_forces(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, int ntiles) {
-    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
-        const int t = tile_index(tb, ntiles);
-        const Tile tile = tiles[t];
+    constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
+    const int nv = ntiles * SUB;
+    for (int vb = blockIdx.x; vb < nv; vb += gridDim.x) {
+        const int v = tile_index(vb, nv);
+        const Tile tile = tiles[v / SUB];
+        const int first = tile.begin + (v % SUB) * BS;
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
         if (((s.flags & FLAG_THERMAL) != 0) != THERMAL) continue;
         double dmax = 0.0;
-        for_work<SCAN>(s, dyn, tile, [&](int i) {
+        for_work<SCAN>(s, dyn, tile, first, [&](int i) {
             if (!SCAN && overflowed(s, i)) return;
             const double d2 =
                 forces_body<D, K, T, THERMAL, SCAN, DEPTH>(s, s.set[cur_set(dyn)], s.set[next_set(dyn)], i,
@@ -415,17 +421,20 @@ __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value && !THERMAL && 
 // ReconstructValue.jl:22-57): Shepard filter over the step-start pairs with
 // the post-continuity density, in binary64 in every form.
 // ---------------------------------------------------------------------------
-template <int D, int K, bool SCAN, int DEPTH>
-__global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__ scenes,
+template <int D, int K, bool SCAN, int DEPTH, int BS>
+__global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* __restrict__ scenes,
                                                     const Tile* __restrict__ tiles, int ntiles) {
     using KQ = KernelQ<double, K>;
-    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
-        const int t = tile_index(tb, ntiles);
-        const Tile tile = tiles[t];
+    constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
+    const int nv = ntiles * SUB;
+    for (int vb = blockIdx.x; vb < nv; vb += gridDim.x) {
+        const int v = tile_index(vb, nv);
+        const Tile tile = tiles[v / SUB];
+        const int first = tile.begin + (v % SUB) * BS;
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
         if (s.kreinit <= 0 || (dyn.step + 1) % s.kreinit != 0) continue;
-        for_work<SCAN>(s, dyn, tile, [&](int i) {
+        for_work<SCAN>(s, dyn, tile, first, [&](int i) {
         if (!SCAN && overflowed(s, i)) return;
         const FluidSet& S = s.set[cur_set(dyn)];
         const int8_t* kind = S.kind;
@@ -469,16 +478,33 @@ __global__ __launch_bounds__(kBlock) void k_shepard(const SceneDev* __restrict__
 }
 
 
+// The pair passes run in workgroups of b.bs threads (the tuner's choice of
+// 64, 128 or 256), kBlock / b.bs of them per particle tile; the overflow
+// (SCAN) instances always run kBlock-thread workgroups.
+#define ESPH_PAIR_LAUNCH(KERNEL_T, ...)                                                          \
+    do {                                                                                        \
+        if constexpr (S) {                                                                      \
+            constexpr int BSZ = kBlock;                                                         \
+            hipLaunchKernelGGL((KERNEL_T), dim3(nt), dim3(BSZ), 0, st, __VA_ARGS__);            \
+        } else {                                                                                \
+            ESPH_DISPATCH_BS(b.bs, {                                                            \
+                hipLaunchKernelGGL((KERNEL_T), dim3(nt * (kBlock / BSZ)), dim3(BSZ), 0, st,      \
+                                   __VA_ARGS__);                                                \
+            });                                                                                 \
+        }                                                                                       \
+    } while (0)
+
 template <int D>
 hipError_t launch_continuity_d(const Bucket& b, hipStream_t st) {
-    dim3 blk(kBlock);
     hipError_t e = hipSuccess;
     ESPH_DISPATCH_K(b.kernel, {
-        e = launch_both(b, st, [&](auto scan, dim3 g, const Tile* tl, int nt) {
+        e = launch_both(b, st, [&](auto scan, const Tile* tl, int nt) {
             constexpr bool S = decltype(scan)::value;
             ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
-                if (b.prec == ESPH_FP64) hipLaunchKernelGGL((k_continuity<D, K, double, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
-                else hipLaunchKernelGGL((k_continuity<D, K, float, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
+                if (b.prec == ESPH_FP64)
+                    ESPH_PAIR_LAUNCH(SINGLE_ARG(k_continuity<D, K, double, S, DP, BSZ>), b.scenes, tl, nt);
+                else
+                    ESPH_PAIR_LAUNCH(SINGLE_ARG(k_continuity<D, K, float, S, DP, BSZ>), b.scenes, tl, nt);
             });
         });
     });
@@ -487,14 +513,15 @@ hipError_t launch_continuity_d(const Bucket& b, hipStream_t st) {
 
 template <int D, bool TH>
 hipError_t launch_forces_t(const Bucket& b, hipStream_t st) {
-    dim3 blk(kBlock);
     hipError_t e = hipSuccess;
     ESPH_DISPATCH_K(b.kernel, {
-        e = launch_both(b, st, [&](auto scan, dim3 g, const Tile* tl, int nt) {
+        e = launch_both(b, st, [&](auto scan, const Tile* tl, int nt) {
             constexpr bool S = decltype(scan)::value;
             ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
-                if (b.prec == ESPH_FP64) hipLaunchKernelGGL((k_forces<D, K, double, TH, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
-                else hipLaunchKernelGGL((k_forces<D, K, float, TH, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
+                if (b.prec == ESPH_FP64)
+                    ESPH_PAIR_LAUNCH(SINGLE_ARG(k_forces<D, K, double, TH, S, DP, BSZ>), b.scenes, tl, nt);
+                else
+                    ESPH_PAIR_LAUNCH(SINGLE_ARG(k_forces<D, K, float, TH, S, DP, BSZ>), b.scenes, tl, nt);
             });
         });
     });
@@ -513,13 +540,12 @@ hipError_t launch_forces_d(const Bucket& b, hipStream_t st) {
 
 template <int D>
 hipError_t launch_shepard_d(const Bucket& b, hipStream_t st) {
-    dim3 blk(kBlock);
     hipError_t e = hipSuccess;
     ESPH_DISPATCH_K(b.kernel, {
-        e = launch_both(b, st, [&](auto scan, dim3 g, const Tile* tl, int nt) {
+        e = launch_both(b, st, [&](auto scan, const Tile* tl, int nt) {
             constexpr bool S = decltype(scan)::value;
             ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
-                hipLaunchKernelGGL((k_shepard<D, K, S, DP>), g, blk, 0, st, b.scenes, tl, nt);
+                ESPH_PAIR_LAUNCH(SINGLE_ARG(k_shepard<D, K, S, DP, BSZ>), b.scenes, tl, nt);
             });
         });
     });

@@ -20,12 +20,15 @@ struct esph_ctx {
     uint64_t budget = 0;           // 0: unlimited
     uint64_t used = 0;
     std::string err;
-    // gather depth of the pair passes: tuned per bucket class (dim, kernel,
-    // precision, thermal mix) at its first step, or fixed
+    // gather depth and workgroup size of the pair passes: tuned per bucket
+    // class (dim, kernel, precision, thermal mix) at its first step, or fixed
     int tune = 1;
     int fixed_depth = 2;
+    int fixed_block = 256;
     std::map<std::array<int, 4>, int> depth;
-    std::map<std::array<int, 4>, std::array<float, 3>> depth_ms;  // forces pass at depth 1, 2, 4
+    std::map<std::array<int, 4>, std::array<float, 3>> depth_ms;  // forces pass at depth 1, 2, 4 (256 threads)
+    std::map<std::array<int, 4>, int> block;
+    std::map<std::array<int, 4>, std::array<float, 3>> block_ms;  // forces pass at 64, 128, 256 threads
 };
 
 namespace esph {
@@ -68,19 +71,21 @@ struct StepPlan {
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<int> captured_depth;  // bucket depths the graph was captured with
+    std::vector<int> captured_depth;  // bucket depths and workgroup sizes the graph was captured with
     int build(esph_ctx* ctx, const std::vector<esph_scene*>& members);
     hipError_t launch_step(hipStream_t st) const;
     int step(esph_ctx* ctx, int64_t nsteps);
     void release();
 };
 
-// Gather-depth tuner.  apply_depth sets b.depth from the context (fixed
-// depth, or the tuned value of b's class; 0 when that class is not tuned
-// yet).  tune_depth times the force pass of b at depths 1, 2 and 4 on the
-// current step's lists (the pass only writes the step outputs, so repeating
-// it changes nothing) and records the fastest; it must follow the step's
-// continuity pass on the stream.  Results do not depend on the depth.
+// Pair-pass tuner.  apply_depth sets b.depth and b.bs from the context
+// (fixed values, or the tuned ones of b's class; depth 0 when that class is
+// not tuned yet).  tune_depth times the force pass of b at gather depths 1,
+// 2 and 4 with 256-thread workgroups, then at the fastest depth with 64- and
+// 128-thread workgroups, on the current step's lists (the pass only writes
+// the step outputs, so repeating it changes nothing), and records the
+// fastest pair; it must follow the step's continuity pass on the stream.
+// Results depend on neither choice.
 std::array<int, 4> depth_key(const Bucket& b);
 void apply_depth(const esph_ctx* ctx, Bucket& b);
 hipError_t tune_depth(esph_ctx* ctx, Bucket& b, hipStream_t st);

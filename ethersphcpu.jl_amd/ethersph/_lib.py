@@ -93,6 +93,9 @@ def lib() -> ctypes.CDLL:
         "esph_last_error": ([vp], c_char_p),
         "esph_ctx_memory_used": ([vp, POINTER(c_uint64)], c_int32),
         "esph_ctx_set_tuning": ([vp, c_int32, c_int32], c_int32),
+        "esph_ctx_set_workgroup": ([vp, c_int32], c_int32),
+        "esph_ctx_tuned_workgroup": ([vp, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_float)],
+                                     c_int32),
         "esph_ctx_tuned_depth": ([vp, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_float)],
                                  c_int32),
         "esph_kernel_eval": ([vp, c_int32, c_int32, c_double, c_int64, DP, DP, DP], c_int32),
@@ -172,9 +175,24 @@ class Context:
             raise EsphError(rc, self._L.esph_last_error(self.handle).decode())
 
     def set_tuning(self, tune: bool = True, depth: int = 2) -> None:
-        """Gather depth of the pair passes: tuned per scene class at its
-        first step (default), or fixed (1, 2 or 4)."""
+        """Gather depth (and workgroup size) of the pair passes: tuned per
+        scene class at its first step (default), or fixed (depth 1, 2 or 4;
+        workgroup size from set_workgroup)."""
         self.check(self._L.esph_ctx_set_tuning(self.handle, 1 if tune else 0, depth))
+
+    def set_workgroup(self, threads: int = 256) -> None:
+        """Workgroup size of the pair passes (64, 128 or 256 threads) when
+        tuning is off."""
+        self.check(self._L.esph_ctx_set_workgroup(self.handle, threads))
+
+    def tuned_workgroup(self, dim: int, kernel: int, precision: int, thermal_mix: int = 1):
+        """(threads, [ms at 64, 128, 256 threads]) of a tuned class, or None."""
+        t = c_int32()
+        ms = (c_float * 3)()
+        rc = self._L.esph_ctx_tuned_workgroup(self.handle, dim, kernel, precision, thermal_mix, ctypes.byref(t), ms)
+        if rc != ESPH_OK:
+            return None
+        return int(t.value), [float(v) for v in ms]
 
     def tuned_depth(self, dim: int, kernel: int, precision: int, thermal_mix: int = 1):
         """(depth, [ms at depth 1, 2, 4]) of a tuned class, or None."""

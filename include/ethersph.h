@@ -238,20 +238,30 @@ int esph_abi_version(void);
 int esph_ctx_create(int32_t device, uint64_t hbm_budget_bytes, esph_ctx** out);
 int esph_ctx_destroy(esph_ctx* ctx);
 const char* esph_last_error(const esph_ctx* ctx);
-/* Bytes of device memory currently held by scenes of this context. */
-/* Gather-depth tuner (no counterpart in the reference: a tuning knob of this
+/* Pair-pass tuner (no counterpart in the reference: tuning knobs of this
    library).  The pair passes keep 1, 2 or 4 neighbour gathers in flight per
-   thread; results are identical at every depth.  mode 1 (default): the first
-   step of each class of scenes (dim, kernel, precision, thermal mix) times
-   the force pass at the three depths on that step's neighbour lists and keeps
-   the fastest; mode 0: the given depth everywhere.  Clears earlier choices;
-   applies to steps launched afterwards. */
+   thread and run in workgroups of 64, 128 or 256 threads; results are
+   identical for every choice.  mode 1 (default): the first step of each class
+   of scenes (dim, kernel, precision, thermal mix) times the force pass at the
+   three depths (256 threads), then at the fastest depth with 64 and 128
+   threads, on that step's neighbour lists, and keeps the fastest; mode 0: the
+   given depth (and the workgroup size of esph_ctx_set_workgroup, 256 by
+   default) everywhere.  Clears earlier choices; applies to steps launched
+   afterwards. */
 int esph_ctx_set_tuning(esph_ctx* ctx, int32_t mode, int32_t depth);
+/* The workgroup size (64, 128 or 256 threads) used in tuning mode 0. */
+int esph_ctx_set_workgroup(esph_ctx* ctx, int32_t threads);
 /* The depth chosen for a class (thermal_mix: 1 plain scenes, 2 thermal, 3
    both in one batch bucket) and, if ms3 is not NULL, the force-pass times
    (ms) measured at depths 1, 2, 4.  ESPH_ERR_STATE if not tuned yet. */
 int esph_ctx_tuned_depth(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision, int32_t thermal_mix,
                          int32_t* depth, float* ms3);
+/* The workgroup size chosen for a class and, if ms3 is not NULL, the
+   force-pass times (ms) at 64, 128 and 256 threads (at the chosen depth).
+   ESPH_ERR_STATE if not tuned yet. */
+int esph_ctx_tuned_workgroup(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision,
+                             int32_t thermal_mix, int32_t* threads, float* ms3);
+/* Bytes of device memory currently held by scenes of this context. */
 int esph_ctx_memory_used(const esph_ctx* ctx, uint64_t* bytes);
 
 /* ---- kernel evaluation (SmoothKernel.jl:37-48 + kernelValue/kernelGradient)

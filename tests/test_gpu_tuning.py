@@ -1,6 +1,7 @@
-"""GPU: the gather-depth tuner.  Results are bit-identical at every depth
-(the pair passes visit the same neighbours in the same order), the tuned
-choice is recorded per scene class, and batches and graphs use it."""
+"""GPU: the pair-pass tuner (gather depth and workgroup size).  Results are
+bit-identical for every choice (the pair passes visit the same neighbours in
+the same order), the tuned choice is recorded per scene class, and batches
+and graphs use it."""
 import sys
 
 import numpy as np
@@ -19,9 +20,10 @@ def say(*a):
     sys.stdout.flush()
 
 
-def run(sc, prec, steps, tune, depth, graph=False):
+def run(sc, prec, steps, tune, depth, graph=False, threads=256):
     ctx = _lib.Context(0)
     ctx.set_tuning(tune, depth)
+    ctx.set_workgroup(threads)
     s = sc.make_scene(prec, ctx)
     if graph:
         s.enable_graph(True)
@@ -47,6 +49,10 @@ def test_results_do_not_depend_on_depth(name, prec):
         got, _ = run(sc, prec, 40, False, depth)
         for f in ref:
             assert np.array_equal(got[f], ref[f]), (depth, f)
+    for threads in (64, 128):
+        got, _ = run(sc, prec, 40, False, 4, threads=threads)
+        for f in ref:
+            assert np.array_equal(got[f], ref[f]), (threads, f)
     got, tuned = run(sc, prec, 40, True, 2)
     assert tuned is not None and tuned[0] in (1, 2, 4) and all(t > 0 for t in tuned[1])
     say("tuned", tuned)
@@ -67,6 +73,8 @@ def test_batch_tunes_each_class_once():
     b.sync()
     for sc in scs:
         assert ctx.tuned_depth(sc.dim, sc.kernel.kind, _lib.FP32_BULK, 1) is not None
+        wg = ctx.tuned_workgroup(sc.dim, sc.kernel.kind, _lib.FP32_BULK, 1)
+        assert wg is not None and wg[0] in (64, 128, 256) and all(t > 0 for t in wg[1])
     b.close()
     for s in scenes:
         s.close()
