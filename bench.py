@@ -79,6 +79,9 @@ def time_gpu(sc, prec, steps, warmup, phase_steps=200):
     tuned = dev.ctx.tuned_depth(sc.dim, sc.kernel.kind, prec, 1)
     if tuned is not None:
         ph["gather_depth"] = {"chosen": tuned[0], "forces_ms_at_depth_1_2_4": tuned[1]}
+    wg = dev.ctx.tuned_workgroup(sc.dim, sc.kernel.kind, prec, 1)
+    if wg is not None:
+        ph["workgroup"] = {"chosen": wg[0], "forces_ms_at_64_128_256_threads": wg[1]}
     n = dev.fluid_count()
     dev.close()
     return wall, ph, n
