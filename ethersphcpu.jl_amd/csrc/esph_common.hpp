@@ -253,15 +253,33 @@ __device__ __forceinline__ void for_pairs(const SceneDev& s, const int32_t* __re
     }
 }
 
+// Returns j once the values p[0..D) -- positions loaded before -- have
+// arrived (an empty asm statement that reads them and rewrites j; no
+// instruction is emitted).  The pair walks gather a group's positions first
+// and its records with the indices returned here, so the records form a
+// second group of loads issued after the first has landed.  Measured on the
+// 1M-particle 3D dam break, FP32 continuity: 0.226 ms against 0.265 with
+// both groups issued together (the compiler then issues part of them after
+// a wait that drains every gather in flight).
+template <int D, typename P>
+__device__ __forceinline__ int after(int j, const P* p) {
+    if (D == 3)
+        asm volatile("" : "+v"(j) : "v"(p[0]), "v"(p[1]), "v"(p[2]));
+    else
+        asm volatile("" : "+v"(j) : "v"(p[0]), "v"(p[1]));
+    return j;
+}
+
 // Pipelined form of for_pairs: neighbours are taken DEPTH at a time; the
-// positions and the records load(j) of a group are all issued before any of
-// their arithmetic, so each lane keeps DEPTH gathers in flight.  The list
+// positions of a group, then its records load(j) (see after; with SPLIT
+// false, together), are issued before any of their arithmetic, so each lane
+// keeps DEPTH gathers in flight.  The list
 // chunk after the current one is fetched while the current one is used.
 // Every slot of a written chunk holds a valid particle index (slots past the
 // count keep earlier entries or zero), so a group may load past the count;
 // body runs only for the counted entries, in list order, so the result does
 // not depend on DEPTH.
-template <int D, bool SCAN, int DEPTH, typename Load, typename Body>
+template <int D, bool SCAN, int DEPTH, bool SPLIT = true, typename Load, typename Body>
 __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __restrict__ cstart,
                                             const double* __restrict__ xall, const double* xi, int i, int self,
                                             const int32_t* __restrict__ list, int cnt, Load&& load, Body&& body) {
@@ -288,10 +306,9 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
                 double xj[DEPTH][3];
                 decltype(load(0)) rec[DEPTH];
 #pragma unroll
-                for (int u = 0; u < DEPTH; ++u) {
-                    load_x<D>(xall, js[g + u], xj[u]);
-                    rec[u] = load(js[g + u]);
-                }
+                for (int u = 0; u < DEPTH; ++u) load_x<D>(xall, js[g + u], xj[u]);
+#pragma unroll
+                for (int u = 0; u < DEPTH; ++u) rec[u] = load(SPLIT ? after<D>(js[g + u], xj[u]) : js[g + u]);
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) {
                     if (g + u < m) {
@@ -385,10 +402,9 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
                 int32_t qj[DEPTH][3];
                 decltype(load(0)) rec[DEPTH];
 #pragma unroll
-                for (int u = 0; u < DEPTH; ++u) {
-                    load_q<D>(S.xq, js[g + u], qj[u]);
-                    rec[u] = load(js[g + u]);
-                }
+                for (int u = 0; u < DEPTH; ++u) load_q<D>(S.xq, js[g + u], qj[u]);
+#pragma unroll
+                for (int u = 0; u < DEPTH; ++u) rec[u] = load(after<D>(js[g + u], qj[u]));
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) {
                     if (g + u < m) {

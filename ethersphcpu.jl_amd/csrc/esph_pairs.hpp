@@ -204,8 +204,11 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     };
     // candidates beyond R contribute exactly zero (weighted, not branched)
     const double R2 = s.R2;
-    for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
-                                [&](int, double d2, const double* rv, const Rec<T>& o) {
+    // the float form gathers positions and records together (measured 2%
+    // faster there; the other walks split them)
+    constexpr bool SPLIT = sizeof(T) == 8;
+    for_pairs_d<D, SCAN, DEPTH, SPLIT>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
+                                       [&](int, double d2, const double* rv, const Rec<T>& o) {
         const T in = d2 <= R2 ? T(1) : T(0);
         const T r = psqrt((T)d2);
         T W, dW;
@@ -250,7 +253,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
             return w;
         };
         const double R2 = s.R2;
-        for_pairs_d<D, SCAN, DEPTH>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], wload,
+        for_pairs_d<D, SCAN, DEPTH, SPLIT>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], wload,
                                     [&](int j, double d2, const double* rv, const WRec& w) {
             // candidates within Rv: the exact rule selects the terms (no branch)
             const bool in = d2 <= R2;
