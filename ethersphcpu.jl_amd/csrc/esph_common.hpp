@@ -270,6 +270,14 @@ __device__ __forceinline__ int after(int j, const P* p) {
     return j;
 }
 
+// Loaders may gather a record in two groups: operator()(j) issues the first
+// part with the positions, late(j, rec) the rest once the first part has
+// arrived (see RecLoaderAB).
+template <typename L, typename = void>
+struct has_late : std::false_type {};
+template <typename L>
+struct has_late<L, std::void_t<decltype(&L::late)>> : std::true_type {};
+
 // Pipelined form of for_pairs: neighbours are taken DEPTH at a time; the
 // positions of a group, then its records load(j) (see after; with SPLIT
 // false, together), are issued before any of their arithmetic, so each lane
@@ -287,8 +295,11 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
     if (SCAN) {
         int c[3];
         cell_of<D>(s, xi, c);
-        for_each_neighbour<D>(s, cstart, xall, xi, c, self,
-                              [&](int j, double d2, const double* rv) { body(j, d2, rv, load(j)); });
+        for_each_neighbour<D>(s, cstart, xall, xi, c, self, [&](int j, double d2, const double* rv) {
+            auto r = load(j);
+            if constexpr (has_late<std::decay_t<Load>>::value) load.late(j, r);
+            body(j, d2, rv, r);
+        });
         return;
     }
     if (cnt <= 0) return;
@@ -309,6 +320,10 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
                 for (int u = 0; u < DEPTH; ++u) load_x<D>(xall, js[g + u], xj[u]);
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) rec[u] = load(SPLIT ? after<D>(js[g + u], xj[u]) : js[g + u]);
+                if constexpr (has_late<std::decay_t<Load>>::value) {
+#pragma unroll
+                    for (int u = 0; u < DEPTH; ++u) load.late(js[g + u], rec[u]);
+                }
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) {
                     if (g + u < m) {
@@ -484,6 +499,38 @@ __device__ __forceinline__ void load_rec(const FluidSet& S, int j, Rec<T>& r) {
         r.kap = b.w;
     }
 }
+
+// load_rec in two groups: the A record (velocity, mass) with the positions,
+// then the B record (density, pressure, gap, temperature) and kappa once A
+// has arrived.  Measured on the float force walk of the 1M-particle 3D dam
+// break: 0.396 ms against 0.403 with one group.
+template <int D, typename T, bool THERMAL>
+struct RecLoaderAB {
+    const FluidSet& S;
+    __device__ __forceinline__ Rec<T> operator()(int j) const {
+        Rec<T> r;
+        load_A<D, T>(S.A, j, r.v, r.m, r.gap);
+        return r;
+    }
+    __device__ __forceinline__ void late(int j, Rec<T>& r) const {
+        using V = typename Vec4<T>::type;
+        if (D == 3)
+            asm volatile("" : "+v"(j) : "v"(r.v[0]), "v"(r.v[1]), "v"(r.v[2]), "v"(r.m));
+        else
+            asm volatile("" : "+v"(j) : "v"(r.v[0]), "v"(r.v[1]), "v"(r.m), "v"(r.gap));
+        const V b = static_cast<const V*>(S.B)[j];
+        r.rho = b.x;
+        r.pr2 = b.y;
+        if (D == 3) {
+            r.gap = b.z;
+            r.Tt = b.w;
+            if (THERMAL) r.kap = ld<T>(S.kap, j);
+        } else {
+            r.Tt = b.z;
+            r.kap = b.w;
+        }
+    }
+};
 
 // Record fields written by the sort (A, B.zw, kap) for particle p.
 template <typename T>

@@ -204,11 +204,10 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     };
     // candidates beyond R contribute exactly zero (weighted, not branched)
     const double R2 = s.R2;
-    // the float form gathers positions and records together (measured 2%
-    // faster there; the other walks split them)
+    // gather groups: binary64, the positions then the records; float, the
+    // positions with the A records then the B records (RecLoaderAB)
     constexpr bool SPLIT = sizeof(T) == 8;
-    for_pairs_d<D, SCAN, DEPTH, SPLIT>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
-                                       [&](int, double d2, const double* rv, const Rec<T>& o) {
+    auto ff = [&](int, double d2, const double* rv, const Rec<T>& o) {
         const T in = d2 <= R2 ? T(1) : T(0);
         const T r = psqrt((T)d2);
         T W, dW;
@@ -231,7 +230,12 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         for (int k = 0; k < D; ++k) dv[k] += o.m * (ps * (T)rv[k] + vf * (me.v[k] - o.v[k]));
         if (THERMAL)
             dT += T(4) * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e) * in;
-    });
+    };
+    if constexpr (SPLIT)
+        for_pairs_d<D, SCAN, DEPTH, true>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load, ff);
+    else
+        for_pairs_d<D, SCAN, DEPTH, false>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i],
+                                           RecLoaderAB<D, T, THERMAL>{S}, ff);
     if (THERMAL) dT *= prcp(rho_i * (T)S.cp[i]);
     if (s.nw > 0) {
         using V = typename Vec4<T>::type;
