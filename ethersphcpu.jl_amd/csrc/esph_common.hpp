@@ -502,8 +502,9 @@ __device__ __forceinline__ void load_rec(const FluidSet& S, int j, Rec<T>& r) {
 
 // load_rec in two groups: the A record (velocity, mass) with the positions,
 // then the B record (density, pressure, gap, temperature) and kappa once A
-// has arrived.  Measured on the float force walk of the 1M-particle 3D dam
-// break: 0.396 ms against 0.403 with one group.
+// has arrived.  Measured on the force walk of the 1M-particle 3D dam break:
+// float 0.396 ms against 0.403 with one group, binary64 0.643 against 0.653
+// with the positions first and both records after them.
 template <int D, typename T, bool THERMAL>
 struct RecLoaderAB {
     const FluidSet& S;

@@ -197,15 +197,11 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     const T inv_Wgap_i = prcp(W_gap);
     T dv[3] = {T(0), T(0), T(0)};
     T dT = T(0);
-    auto load = [&](int j) {
-        Rec<T> o;
-        load_rec<D, T, THERMAL>(S, j, o);
-        return o;
-    };
     // candidates beyond R contribute exactly zero (weighted, not branched)
     const double R2 = s.R2;
-    // gather groups: binary64, the positions then the records; float, the
-    // positions with the A records then the B records (RecLoaderAB)
+    // gather groups: the positions with the A records, then the B records
+    // (RecLoaderAB); the wall walk below: binary64, the positions then the
+    // records; float, both together
     constexpr bool SPLIT = sizeof(T) == 8;
     auto ff = [&](int, double d2, const double* rv, const Rec<T>& o) {
         const T in = d2 <= R2 ? T(1) : T(0);
@@ -231,11 +227,8 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         if (THERMAL)
             dT += T(4) * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e) * in;
     };
-    if constexpr (SPLIT)
-        for_pairs_d<D, SCAN, DEPTH, true>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load, ff);
-    else
-        for_pairs_d<D, SCAN, DEPTH, false>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i],
-                                           RecLoaderAB<D, T, THERMAL>{S}, ff);
+    for_pairs_d<D, SCAN, DEPTH, false>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i],
+                                       RecLoaderAB<D, T, THERMAL>{S}, ff);
     if (THERMAL) dT *= prcp(rho_i * (T)S.cp[i]);
     if (s.nw > 0) {
         using V = typename Vec4<T>::type;
