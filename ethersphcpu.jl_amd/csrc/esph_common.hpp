@@ -396,7 +396,8 @@ __device__ __forceinline__ bool qgeom(const SceneDev& s, const double* __restric
 // The list walk of for_pairs_d with the candidates' geometry from their
 // quantized positions (one 16-byte or 8-byte load instead of the binary64
 // position): body(j, in, d2, rv, record) with in = the FP64 rule's verdict.
-template <int D, int DEPTH, typename Load, typename Body>
+// Gather groups as in for_pairs_d (SPLIT: the positions, then the records).
+template <int D, int DEPTH, bool SPLIT = true, typename Load, typename Body>
 __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S, const double* xi, int i,
                                             const int32_t* __restrict__ list, int cnt, Load&& load, Body&& body) {
     static_assert(DEPTH == 1 || DEPTH == 2 || DEPTH == 4, "DEPTH divides a 4-entry chunk");
@@ -419,7 +420,11 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) load_q<D>(S.xq, js[g + u], qj[u]);
 #pragma unroll
-                for (int u = 0; u < DEPTH; ++u) rec[u] = load(after<D>(js[g + u], qj[u]));
+                for (int u = 0; u < DEPTH; ++u) rec[u] = load(SPLIT ? after<D>(js[g + u], qj[u]) : js[g + u]);
+                if constexpr (has_late<std::decay_t<Load>>::value) {
+#pragma unroll
+                    for (int u = 0; u < DEPTH; ++u) load.late(js[g + u], rec[u]);
+                }
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) {
                     if (g + u < m) {
@@ -503,8 +508,10 @@ __device__ __forceinline__ void load_rec(const FluidSet& S, int j, Rec<T>& r) {
 // load_rec in two groups: the A record (velocity, mass) with the positions,
 // then the B record (density, pressure, gap, temperature) and kappa once A
 // has arrived.  Measured on the force walk of the 1M-particle 3D dam break:
-// float 0.396 ms against 0.403 with one group, binary64 0.643 against 0.653
-// with the positions first and both records after them.
+// binary64 0.643 ms against 0.653 with the positions first and both records
+// after them; float (quantized positions) 0.343 against 0.353 in three
+// groups (positions, A, B) and 0.395 for the binary64 positions with one
+// group.
 template <int D, typename T, bool THERMAL>
 struct RecLoaderAB {
     const FluidSet& S;
