@@ -154,19 +154,38 @@ int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
     }
     plan = SortPlan{dtable, n, dtiles, (int)all_p.size(), dtiles + all_p.size(), (int)all_c.size()};
     nmembers = n;
+    this->members = members;
     return ESPH_OK;
 }
 
-hipError_t StepPlan::launch_step(hipStream_t st) const {
+hipError_t StepPlan::launch_step(hipStream_t st, bool shepard) const {
     hipError_t e = hipSuccess;
     for (const Bucket& b : buckets)
         if (e == hipSuccess) e = launch_continuity(b, st);
     for (const Bucket& b : buckets)
         if (e == hipSuccess) e = launch_forces(b, st);
-    for (const Bucket& b : buckets)  // gated per scene on the device
-        if (e == hipSuccess) e = launch_shepard(b, st);
+    if (shepard)
+        for (const Bucket& b : buckets)  // gated per scene on the device
+            if (e == hipSuccess) e = launch_shepard(b, st);
     if (e == hipSuccess) e = launch_sort(plan, 0, st);
     return e;
+}
+
+bool StepPlan::shepard_due(int64_t t) const {
+    for (const esph_scene* s : members) {
+        const int64_t K = s->d.density_reinitialized_interval;
+        if (K > 0 && (s->step + t + 1) % K == 0) return true;
+    }
+    return false;
+}
+
+void StepPlan::drop_graphs() {
+    for (int q = 0; q < 2; ++q) {
+        if (exec[q]) (void)hipGraphExecDestroy(exec[q]);
+        if (graph[q]) (void)hipGraphDestroy(graph[q]);
+        exec[q] = nullptr;
+        graph[q] = nullptr;
+    }
 }
 
 int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
@@ -178,6 +197,7 @@ int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
         apply_depth(ctx, b);
         untuned = untuned || b.depth == 0;
     }
+    int64_t t0 = 0;  // steps of this call already launched
     if (untuned && nsteps > 0) {
         hipError_t e = hipSuccess;
         for (const Bucket& b : buckets)
@@ -189,11 +209,13 @@ int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
             }
         for (const Bucket& b : buckets)
             if (e == hipSuccess) e = launch_forces(b, st);
-        for (const Bucket& b : buckets)
-            if (e == hipSuccess) e = launch_shepard(b, st);
+        if (shepard_due(0))
+            for (const Bucket& b : buckets)
+                if (e == hipSuccess) e = launch_shepard(b, st);
         if (e == hipSuccess) e = launch_sort(plan, 0, st);
         if (e != hipSuccess) return hip_fail(ctx, "tuning step", e);
         nsteps -= 1;
+        t0 = 1;
     }
     // a graph captured with other depths or workgroup sizes (tuning settings
     // changed) is recaptured
@@ -202,37 +224,31 @@ int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
         depths.push_back(b.depth);
         depths.push_back(b.bs);
     }
-    if (exec && depths != captured_depth) {
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
-    }
-    if (use_graph && !exec && nsteps > 0) {
-        captured_depth = depths;
-        hipGraph_t g = nullptr;
-        HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t e = launch_step(st);
-        hipError_t e2 = hipStreamEndCapture(st, &g);
-        if (e != hipSuccess) return hip_fail(ctx, "graph capture", e);
-        if (e2 != hipSuccess) return hip_fail(ctx, "graph capture end", e2);
-        graph = g;
-        HIPCHK(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
-    for (int64_t t = 0; t < nsteps; ++t) {
-        hipError_t e = use_graph ? hipGraphLaunch(exec, st) : launch_step(st);
+    if (depths != captured_depth) drop_graphs();
+    for (int64_t t = t0; t < t0 + nsteps; ++t) {
+        const int q = shepard_due(t) ? 1 : 0;
+        if (use_graph && !exec[q]) {
+            captured_depth = depths;
+            hipGraph_t g = nullptr;
+            HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipError_t e = launch_step(st, q == 1);
+            hipError_t e2 = hipStreamEndCapture(st, &g);
+            if (e != hipSuccess) return hip_fail(ctx, "graph capture", e);
+            if (e2 != hipSuccess) return hip_fail(ctx, "graph capture end", e2);
+            graph[q] = g;
+            HIPCHK(ctx, hipGraphInstantiate(&exec[q], graph[q], nullptr, nullptr, 0));
+        }
+        hipError_t e = use_graph ? hipGraphLaunch(exec[q], st) : launch_step(st, q == 1);
         if (e != hipSuccess) return hip_fail(ctx, "batch step", e);
     }
     return ESPH_OK;
 }
 
 void StepPlan::release() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
+    drop_graphs();
     arena.release();
     buckets.clear();
+    members.clear();
     nmembers = 0;
 }
 
@@ -282,12 +298,7 @@ extern "C" int esph_batch_destroy(esph_batch* b) {
 
 extern "C" int esph_batch_enable_graph(esph_batch* b, int32_t enable) {
     if (!b) return ESPH_ERR_ARG;
-    if (!enable && b->plan.exec) {
-        (void)hipGraphExecDestroy(b->plan.exec);
-        (void)hipGraphDestroy(b->plan.graph);
-        b->plan.exec = nullptr;
-        b->plan.graph = nullptr;
-    }
+    if (!enable) b->plan.drop_graphs();
     b->plan.use_graph = enable != 0;
     return ESPH_OK;
 }

@@ -63,17 +63,23 @@ struct TimedStep {
 // Launch plan of one step for a set of scenes (a batch, or one scene in
 // graph mode): device scene table, per-bucket tile tables, sort plan and the
 // optionally captured step graph.
+// The Shepard pass is launched only on steps where some member re-initialises
+// its density (the host knows every member's step count), so a plan holds
+// two step graphs, [0] without and [1] with that pass.
 struct StepPlan {
     Arena arena;
     std::vector<Bucket> buckets;
     SortPlan plan{};
     int nmembers = 0;
+    std::vector<esph_scene*> members;
     bool use_graph = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    std::vector<int> captured_depth;  // bucket depths and workgroup sizes the graph was captured with
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    std::vector<int> captured_depth;  // bucket depths and workgroup sizes the graphs were captured with
     int build(esph_ctx* ctx, const std::vector<esph_scene*>& members);
-    hipError_t launch_step(hipStream_t st) const;
+    hipError_t launch_step(hipStream_t st, bool shepard) const;
+    bool shepard_due(int64_t t) const;  // some member's step t from now re-initialises its density
+    void drop_graphs();
     int step(esph_ctx* ctx, int64_t nsteps);
     void release();
 };
