@@ -52,6 +52,7 @@ struct SceneDyn {
     uint32_t disp2_bits;  // float bits of this step's largest squared displacement from xref
     int32_t nsorts;       // sorts (list rebuilds) so far, the upload's included
     int32_t scan_done;    // cell-count tiles scanned in this sort (reset by the last one)
+    int32_t outside_acc;  // particles far outside the grid, counted by the running sort
 };
 
 // Index into SceneDev::set of the current state and of the step's outputs

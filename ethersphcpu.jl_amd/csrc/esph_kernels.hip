@@ -60,14 +60,29 @@ __device__ __forceinline__ int wave_counted_slot(int32_t* count, int key) {
     return base + ((int)lane - my_head);
 }
 
+// Whether this step ends with a sort: always at upload, when the candidate
+// lists are rebuilt every step, on check steps, for the inlet/outlet solver,
+// and when some particle has moved (Rv - R) / 2 since the lists were built
+// (or a position is not finite).  Otherwise the step commits in place.
+// Reads only state that is final once the force pass has run.
+__device__ __forceinline__ int sort_decision(const SceneDev& s, const SceneDyn& d, int mode) {
+    const float disp2 = __uint_as_float(d.disp2_bits);
+    const bool moved = !((double)disp2 <= s.disp2_limit);
+    return (mode != 0 || !s.verlet || (s.flags & FLAG_IO) || check_due(s, d) || moved) ? 1 : 0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict__ scenes,
                                                      const Tile* __restrict__ tiles, int ntiles, int mode) {
     for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
         const int t = tile_index(tb, ntiles);
         const Tile tile = tiles[t];
         const SceneDev& s = scenes[tile.scene];
-        if (!s.dyn->sorted) continue;  // a step without sort
         const SceneDyn dyn = *s.dyn;
+        // every block takes the step's decision itself; the scene's first
+        // tile records it for the later sort kernels
+        const int sorted = sort_decision(s, dyn, mode);
+        if (tile.begin == 0 && threadIdx.x == 0) s.dyn->sorted = sorted;
+        if (!sorted) continue;  // a step without sort
         const int i = tile.begin + (int)threadIdx.x;
         if (i >= dyn.n_live) continue;
         const int D = s.dim;
@@ -112,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
                 const double f = floor((x[k] - s.lo[k]) * s.inv_cs);
                 far = far || !(f >= -1.0 && f <= (double)s.nc[k]);
             }
-            if (far && !dead) atomicAdd(&s.dyn->outside, 1);
+            if (far && !dead) atomicAdd(&s.dyn->outside_acc, 1);
         }
         s.key[i] = key;
         s.slot[i] = wave_counted_slot(s.count, key);
@@ -419,22 +434,6 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
     }
 }
 
-// Whether this step ends with a sort: always at upload, when the candidate
-// lists are rebuilt every step, on check steps, for the inlet/outlet solver,
-// and when some particle has moved (Rv - R) / 2 since the lists were built
-// (or a position is not finite).  Otherwise the step commits in place.
-__global__ void k_sort_decide(const SceneDev* __restrict__ scenes, int nscenes, int mode) {
-    const int sc = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sc >= nscenes) return;
-    const SceneDev& s = scenes[sc];
-    SceneDyn* d = s.dyn;
-    const float disp2 = __uint_as_float(d->disp2_bits);
-    const bool moved = !((double)disp2 <= s.disp2_limit);
-    const int sorted = (mode != 0 || !s.verlet || (s.flags & FLAG_IO) || check_due(s, *d) || moved) ? 1 : 0;
-    d->sorted = sorted;
-    if (sorted) d->outside = 0;  // counted again by this sort
-}
-
 __global__ void k_sort_finish(const SceneDev* __restrict__ scenes, int nscenes, int mode) {
     const int sc = blockIdx.x * blockDim.x + threadIdx.x;
     if (sc >= nscenes) return;
@@ -447,6 +446,8 @@ __global__ void k_sort_finish(const SceneDev* __restrict__ scenes, int nscenes, 
             d->next_outlet_id += d->nleave;
         }
         d->n_live = s.cell_start[s.ncell];
+        d->outside = d->outside_acc;
+        d->outside_acc = 0;
         d->parity ^= 1;  // the sort wrote the other order, per-step copy dpar
         d->rebuild = 1;
         d->nsorts += 1;
@@ -574,7 +575,6 @@ hipError_t launch_shepard(const Bucket& b, hipStream_t st) {
 
 hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st) {
     dim3 blk(kBlock);
-    hipLaunchKernelGGL(k_sort_decide, dim3((p.nscenes + 63) / 64), dim3(64), 0, st, p.scenes, p.nscenes, mode);
     hipLaunchKernelGGL(k_sort_bin, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_scan1, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
     hipLaunchKernelGGL(k_sort_scan3, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
