@@ -42,6 +42,16 @@ __device__ __forceinline__ void load_x(const double* __restrict__ x, int j, doub
     }
 }
 
+// A 4-entry candidate-list chunk, loaded non-temporally: each pass reads a
+// particle's list once, and keeping those lines out of the caches leaves
+// them to the positions and records the pass gathers (measured on the
+// 1M-particle 3D dam break: FP32 step 0.619 -> 0.598 ms, FP64 1.078 -> 1.064).
+typedef int esph_nint4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int4 ld_chunk(const int32_t* p) {
+    const esph_nint4 v = __builtin_nontemporal_load(reinterpret_cast<const esph_nint4*>(p));
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
 // Candidate rows of a point xi in cell c: the grid's cells are at least R /
 // stencil wide, so neighbours lie within `stencil` cells on every axis.  For
 // each (dy, dz) the cells of the row are x-contiguous in memory, so a row is
@@ -305,10 +315,10 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
     if (cnt <= 0) return;
     const size_t cap = (size_t)s.cap;
     const int nq = (cnt + 3) >> 2;
-    int4 cur = *reinterpret_cast<const int4*>(list + list_at(cap, 0, i));
+    int4 cur = ld_chunk(list + list_at(cap, 0, i));
     for (int q = 0; q < nq; ++q) {
         int4 nxt = cur;
-        if (q + 1 < nq) nxt = *reinterpret_cast<const int4*>(list + list_at(cap, 4 * (q + 1), i));
+        if (q + 1 < nq) nxt = ld_chunk(list + list_at(cap, 4 * (q + 1), i));
         const int js[4] = {cur.x, cur.y, cur.z, cur.w};
         const int m = cnt - 4 * q;  // entries of this chunk: min(4, m)
 #pragma unroll
@@ -406,10 +416,10 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
     load_q<D>(S.xq, i, qi);
     const size_t cap = (size_t)s.cap;
     const int nq = (cnt + 3) >> 2;
-    int4 cur = *reinterpret_cast<const int4*>(list + list_at(cap, 0, i));
+    int4 cur = ld_chunk(list + list_at(cap, 0, i));
     for (int q = 0; q < nq; ++q) {
         int4 nxt = cur;
-        if (q + 1 < nq) nxt = *reinterpret_cast<const int4*>(list + list_at(cap, 4 * (q + 1), i));
+        if (q + 1 < nq) nxt = ld_chunk(list + list_at(cap, 4 * (q + 1), i));
         const int js[4] = {cur.x, cur.y, cur.z, cur.w};
         const int m = cnt - 4 * q;  // entries of this chunk: min(4, m)
 #pragma unroll
