@@ -98,8 +98,9 @@ struct FluidSet {
     // FP32_BULK only (null otherwise): positions relative to the grid origin
     // in whole quanta SceneDev::qu (int4 in 3D with w = 0, int2 in 2D; every
     // component kQInvalid when a coordinate is beyond 2^30 quanta or not
-    // finite).  Differences of these are exact, so the continuity pass takes
-    // its pair geometry from one 16-byte load per candidate (see qgeom).
+    // finite).  Differences of these are exact, so the continuity pass and
+    // the 3D force pass's fluid walk take their pair geometry from one
+    // 16-byte load per candidate (see qgeom).
     void* xq;
 };
 
