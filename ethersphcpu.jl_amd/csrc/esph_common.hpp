@@ -416,6 +416,37 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
     load_q<D>(S.xq, i, qi);
     const size_t cap = (size_t)s.cap;
     const int nq = (cnt + 3) >> 2;
+    if constexpr (D == 3 && DEPTH == 4 && SPLIT) {
+        // the 3D continuity walk at depth 4 takes two chunks (8 candidates)
+        // per group: 0.208 ms against 0.219 on the 1M-particle dam break (in
+        // 2D, with shorter lists, the batched ensemble measured 10% slower)
+        int4 c0 = ld_chunk(list + list_at(cap, 0, i));
+        int4 c1 = nq > 1 ? ld_chunk(list + list_at(cap, 4, i)) : c0;
+        for (int q = 0; q < nq; q += 2) {
+            int4 n0 = c0, n1 = c1;
+            if (q + 2 < nq) n0 = ld_chunk(list + list_at(cap, 4 * (q + 2), i));
+            if (q + 3 < nq) n1 = ld_chunk(list + list_at(cap, 4 * (q + 3), i));
+            const int js[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            const int m = cnt - 4 * q;  // entries of these two chunks: min(8, m)
+            int32_t qj[8][3];
+            decltype(load(0)) rec[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) load_q<D>(S.xq, js[u], qj[u]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rec[u] = load(after<D>(js[u], qj[u]));
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (u < m) {
+                    float rv[3], d2;
+                    const bool in = qgeom<D>(s, S.x, xi, qi, qj[u], js[u], rv, d2);
+                    body(js[u], in, d2, rv, rec[u]);
+                }
+            }
+            c0 = n0;
+            c1 = n1;
+        }
+        return;
+    }
     int4 cur = ld_chunk(list + list_at(cap, 0, i));
     for (int q = 0; q < nq; ++q) {
         int4 nxt = cur;
