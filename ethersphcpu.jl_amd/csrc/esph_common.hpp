@@ -288,7 +288,8 @@ struct has_late : std::false_type {};
 template <typename L>
 struct has_late<L, std::void_t<decltype(&L::late)>> : std::true_type {};
 
-// Pipelined form of for_pairs: neighbours are taken DEPTH at a time; the
+// Pipelined form of for_pairs: neighbours are taken DEPTH at a time (WIDE
+// with DEPTH 4: two chunks, 8 candidates, at a time); the
 // positions of a group, then its records load(j) (see after; with SPLIT
 // false, together), are issued before any of their arithmetic, so each lane
 // keeps DEPTH gathers in flight.  The list
@@ -297,7 +298,7 @@ struct has_late<L, std::void_t<decltype(&L::late)>> : std::true_type {};
 // count keep earlier entries or zero), so a group may load past the count;
 // body runs only for the counted entries, in list order, so the result does
 // not depend on DEPTH.
-template <int D, bool SCAN, int DEPTH, bool SPLIT = true, typename Load, typename Body>
+template <int D, bool SCAN, int DEPTH, bool SPLIT = true, bool WIDE = false, typename Load, typename Body>
 __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __restrict__ cstart,
                                             const double* __restrict__ xall, const double* xi, int i, int self,
                                             const int32_t* __restrict__ list, int cnt, Load&& load, Body&& body) {
@@ -315,6 +316,39 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
     if (cnt <= 0) return;
     const size_t cap = (size_t)s.cap;
     const int nq = (cnt + 3) >> 2;
+    if constexpr (WIDE && DEPTH == 4) {
+        // two chunks (8 candidates) per group, as in for_pairs_q
+        int4 c0 = ld_chunk(list + list_at(cap, 0, i));
+        int4 c1 = nq > 1 ? ld_chunk(list + list_at(cap, 4, i)) : c0;
+        for (int q = 0; q < nq; q += 2) {
+            int4 n0 = c0, n1 = c1;
+            if (q + 2 < nq) n0 = ld_chunk(list + list_at(cap, 4 * (q + 2), i));
+            if (q + 3 < nq) n1 = ld_chunk(list + list_at(cap, 4 * (q + 3), i));
+            const int js[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            const int m = cnt - 4 * q;  // entries of these two chunks: min(8, m)
+            double xj[8][3];
+            decltype(load(0)) rec[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) load_x<D>(xall, js[u], xj[u]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rec[u] = load(SPLIT ? after<D>(js[u], xj[u]) : js[u]);
+            if constexpr (has_late<std::decay_t<Load>>::value) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) load.late(js[u], rec[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (u < m) {
+                    double rv[3];
+                    const double d2 = dist2_exact<D>(xi, xj[u], rv);
+                    body(js[u], d2, rv, rec[u]);
+                }
+            }
+            c0 = n0;
+            c1 = n1;
+        }
+        return;
+    }
     int4 cur = ld_chunk(list + list_at(cap, 0, i));
     for (int q = 0; q < nq; ++q) {
         int4 nxt = cur;
@@ -419,7 +453,10 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
     if constexpr (D == 3 && DEPTH == 4 && SPLIT) {
         // the 3D continuity walk at depth 4 takes two chunks (8 candidates)
         // per group: 0.208 ms against 0.219 on the 1M-particle dam break (in
-        // 2D, with shorter lists, the batched ensemble measured 10% slower)
+        // 2D, with shorter lists, the batched ensemble measured 10% slower).
+        // The binary64 walks of the continuity and Shepard passes do the
+        // same in 3D (for_pairs_d, WIDE): 0.362 against 0.375 ms and 0.290
+        // against 0.306.
         int4 c0 = ld_chunk(list + list_at(cap, 0, i));
         int4 c1 = nq > 1 ? ld_chunk(list + list_at(cap, 4, i)) : c0;
         for (int q = 0; q < nq; q += 2) {

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
                 pair(in ? T(1) : T(0), d2, rv, a);
             });
         } else {
-            for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
+            for_pairs_d<D, SCAN, DEPTH, true, D == 3>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
                                         [&](int, double d2, const double* rv, const ARec<T>& a) {
                 T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
                 pair(d2 <= R2 ? T(1) : T(0), (T)d2, rvt, a);
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* 
             return r;
         };
         const double R2 = s.R2;
-        for_pairs_d<D, SCAN, DEPTH>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
+        for_pairs_d<D, SCAN, DEPTH, true, D == 3>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
                                     [&](int, double d2, const double*, const SRec& o) {
             // weight 1 for neighbours under the exact rule among fluid particles
             const double in = d2 <= R2 && o.kind == KIND_FLUID ? 1.0 : 0.0;
