@@ -289,15 +289,15 @@ template <typename L>
 struct has_late<L, std::void_t<decltype(&L::late)>> : std::true_type {};
 
 // Pipelined form of for_pairs: neighbours are taken DEPTH at a time (WIDE
-// with DEPTH 4: two chunks, 8 candidates, at a time); the
-// positions of a group, then its records load(j) (see after; with SPLIT
-// false, together), are issued before any of their arithmetic, so each lane
-// keeps DEPTH gathers in flight.  The list
-// chunk after the current one is fetched while the current one is used.
-// Every slot of a written chunk holds a valid particle index (slots past the
-// count keep earlier entries or zero), so a group may load past the count;
-// body runs only for the counted entries, in list order, so the result does
-// not depend on DEPTH.
+// with DEPTH 4: two chunks, 8 candidates, at a time).  The positions of a
+// group, then its records load(j) (see after; with SPLIT false, together),
+// are issued before any of their arithmetic, so each lane keeps DEPTH
+// gathers in flight.  The list chunk after the current one is fetched while
+// the current one is used.  Every slot of a written chunk holds a valid
+// particle index (slots past the count keep earlier entries or zero), and a
+// group past the last chunk reuses an earlier one, so a group may load past
+// the count; body runs only for the counted entries, in list order, so the
+// result depends neither on DEPTH nor on WIDE.
 template <int D, bool SCAN, int DEPTH, bool SPLIT = true, bool WIDE = false, typename Load, typename Body>
 __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __restrict__ cstart,
                                             const double* __restrict__ xall, const double* xi, int i, int self,
