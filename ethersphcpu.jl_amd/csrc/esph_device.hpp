@@ -92,8 +92,9 @@ struct FluidSet {
     void* B;
     void* kap;
     // positions relative to the grid origin in float (float4 in 3D, float2 in
-    // 2D): the neighbour-list build tests candidates on these and falls back
-    // to the FP64 rule only inside a rounding band around R
+    // 2D), written by the sort for its new order: the neighbour-list build
+    // after it tests candidates on these and falls back to the FP64 rule
+    // only inside a rounding band around R
     void* xf;
     // FP32_BULK only (null otherwise): positions relative to the grid origin
     // in whole quanta SceneDev::qu (int4 in 3D with w = 0, int2 in 2D; every

@@ -135,16 +135,14 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
 //              = 4 k_i k_j r W' (T_i - T_j) m_j / ((k_i + k_j) rho_j (r^2 + .01 h^2)) / (rho_i cp_i)
 // and 1 / W(gap) is taken per particle when both gaps are equal.
 // The new state goes to the other per-step copy N in the current particle
-// order: x, v, T, the float position and whole A and B records (B.xy keep
+// order: x, v, T, the quantized position and whole A and B records (B.xy keep
 // this step's density and pressure until the next continuity pass).
 // ---------------------------------------------------------------------------
+// The float positions (FluidSet::xf) are read only by the list rebuild,
+// which follows a sort, and the sort writes them for its new order; the
+// per-step copies therefore carry the quantized positions alone.
 template <int D>
-__device__ __forceinline__ void store_xf(const SceneDev& s, const FluidSet& N, int i, const double* x) {
-    if (D == 3)
-        static_cast<float4*>(N.xf)[i] = make_float4((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]),
-                                                    (float)(x[2] - s.lo[2]), 0.0f);
-    else
-        static_cast<float2*>(N.xf)[i] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
+__device__ __forceinline__ void store_next_q(const SceneDev& s, const FluidSet& N, int i, const double* x) {
     if (N.xq) store_xq<D>(s, N, i, x);
 }
 
@@ -164,7 +162,7 @@ __device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S
         x[k] = S.x[i * D + k] + (v * dt - 0.0 * (dt * dt) / 2.0);
         N.x[i * D + k] = x[k];
     }
-    store_xf<D>(s, N, i, x);
+    store_next_q<D>(s, N, i, x);
     static_cast<V*>(N.A)[i] = static_cast<const V*>(S.A)[i];
     static_cast<V*>(N.B)[i] = static_cast<const V*>(S.B)[i];
     if (N.T) N.T[i] = S.T[i];
@@ -364,7 +362,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 disp2 += e * e;
             }
         }
-        store_xf<D>(s, N, i, xnew);
+        store_next_q<D>(s, N, i, xnew);
         // records of the new state; float records hold T - t0 (see k_sort_scatter)
         using V = typename Vec4<T>::type;
         const T Tr = THERMAL ? (T)(sizeof(T) == 4 ? Tn - s.t0 : Tn) : me.Tt;
