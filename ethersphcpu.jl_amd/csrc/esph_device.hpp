@@ -11,7 +11,13 @@
 namespace esph {
 
 constexpr int kBlock = 256;          // particles per tile == threads per block
-constexpr int kScanItems = 4;        // cells per thread in the scan kernels
+// cells per thread in the scan kernels: 4,096 cells per scan tile.  Each
+// tile's block fences its total before the ticket of k_sort_scan1, and that
+// fence writes the L2 back, so fewer, larger tiles make a sort cheaper: 1,024
+// cells per tile 0.586 ms per 3D dam-break step, 4,096 0.580, 8,192 0.579
+// (but the 2D ensemble, with smaller grids, 1.78 ms per batched step
+// against 1.77 at 4,096)
+constexpr int kScanItems = 16;
 constexpr int kCellTile = kBlock * kScanItems;
 constexpr int kOvfTiles = 16;        // tiles per scene of the overflow (cell-scan) kernels
 
