@@ -439,7 +439,7 @@ __device__ __forceinline__ bool qgeom(const SceneDev& s, const double* __restric
 
 // The list walk of for_pairs_d with the candidates' geometry from their
 // quantized positions (one 16-byte or 8-byte load instead of the binary64
-// position): body(j, in, d2, rv, record) with in = the FP64 rule's verdict.
+// position): body(j, in, d2, rv, record) with in = the FP64 rule's decision.
 // Gather groups as in for_pairs_d (SPLIT: the positions, then the records).
 template <int D, int DEPTH, bool SPLIT = true, typename Load, typename Body>
 __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S, const double* xi, int i,
