@@ -6,14 +6,17 @@ dx = 0.0089 -> 1,013,040 fluid + 645,804 wall particles, WendlandC4 with
 h = 1.3 dx, linear EOS, Shepard every 30 steps, FP32 pair arithmetic with
 FP64 integration (FP32_BULK).  A "step" is one eachStep! of the whole scene
 with the particle state resident in HBM.  The same scene in the plain FP64
-form is timed beside it, and the FP64 CPU oracle is timed on a bounded sample
-(a few steps of the same scene) on the host cores.
+form is timed beside it over the same window (fp32_over_fp64), and both forms
+again over the workload's 1,000 steps when --steps is another count
+(long_run); the FP64 CPU oracle is timed on a bounded sample (a few steps of
+the same scene) on every host core available to the process.
 
 Two further legs report the other halves of the metric: "ensemble" (256 2D
 scenes, half lid-driven cavities and half Poiseuille channels, sizes
 log-uniform in [2k, 60k], batched with graphs versus launched one scene at a
-time) and, with --serving, "serving" (Poisson arrivals of 2D dam breaks, 3D
-Poiseuille pipes and 2D thermal cavities under a 200 GB HBM budget: completed
+time) and "serving" (Poisson arrivals, faster than the server completes them,
+of 2D dam breaks, 3D Poiseuille pipes and 2D thermal cavities at the
+workload's sizes and step counts under a 200 GB HBM budget: completed
 scenes/s and p50/p99 step latency).  Both are bounded samples, stated in
 their output.
 
@@ -145,7 +148,9 @@ def ensemble_leg(steps, alone_steps, n_scenes, full_steps=5000):
 
 
 def serving_leg(n_requests, rate, step_scale, budget_gb, advance_steps):
-    """Poisson arrivals of mixed requests served by continuous batching."""
+    """Poisson arrivals of mixed requests served by continuous batching.  The
+    arrival rate is set above the server's capacity (the queue never runs
+    dry), so completed scenes/s measures capacity, not the arrival rate."""
     from ethersph import _lib, scenarios
     from ethersph.serving import Server
 
@@ -179,8 +184,33 @@ def serving_leg(n_requests, rate, step_scale, budget_gb, advance_steps):
     return {"requests": n_requests, "completed": int(st["completed"]), "failed": failed,
             "scenes_per_s": st["completed"] / wall, "p50_step_ms": st["p50_step_ms"],
             "p99_step_ms": st["p99_step_ms"], "particle_steps_per_s": psteps / wall, "wall_s": wall,
-            "sample": f"{n_requests} Poisson arrivals at {rate}/s, steps {lo}-{hi} per request "
-                      f"(the workload's 1,000-10,000 scaled by {step_scale}), {budget_gb} GB budget, FP32_BULK"}
+            "last_arrival_s": stream[-1][0], "batch_steps": int(st["batch_steps"]),
+            "sample": f"{n_requests} Poisson arrivals at {rate}/s (all queued within "
+                      f"{stream[-1][0]:.2f} s, above capacity), steps {lo}-{hi} per request "
+                      f"(the workload's 1,000-10,000 x {step_scale}), sizes as the workload names them, "
+                      f"{budget_gb} GB budget, FP32_BULK"}
+
+
+def host_cores():
+    """Threads for the CPU baseline: every core this process may run on
+    (sched affinity), or OMP_NUM_THREADS when the host sets it lower; with
+    the host's core count and CPU model for the record."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else avail
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"nproc": os.cpu_count(), "affinity_cores": avail, "OMP_NUM_THREADS": omp, "cpu_model": model}
 
 
 def cpu_baseline(sc, steps, threads):
@@ -200,19 +230,19 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--dx", type=float, default=0.0089)
-    ap.add_argument("--fp64-steps", type=int, default=300)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--long-steps", type=int, default=1000,
+                    help="both forms over this many steps for the fp32_over_fp64 figure of the configured run")
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-scenes", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=300)
     ap.add_argument("--ensemble-alone-steps", type=int, default=30)
-    ap.add_argument("--serving-requests", type=int, default=24)
-    ap.add_argument("--serving-rate", type=float, default=4.0)
-    ap.add_argument("--serving-step-scale", type=float, default=0.1)
+    ap.add_argument("--serving-requests", type=int, default=16)
+    ap.add_argument("--serving-rate", type=float, default=50.0)
+    ap.add_argument("--serving-step-scale", type=float, default=1.0)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true", help="run only the ensemble leg (for profiling)")
-    ap.add_argument("--serving", action="store_true",
-                    help="also run the bounded serving sample (off by default: see the design notes)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
@@ -229,11 +259,20 @@ def main():
     log(f"workload dam_break_3d: {nf} fluid, {nw} wall; FP32_BULK {a.steps} steps")
     wall32, ph32, n_end = time_gpu(sc, _lib.FP32_BULK, a.steps, a.warmup)
     log(f"FP32_BULK: {wall32 * 1e3 / a.steps:.3f} ms/step, phases {ph32}")
-    wall64, ph64, _ = time_gpu(sc, _lib.FP64, a.fp64_steps, a.warmup)
-    log(f"FP64: {wall64 * 1e3 / a.fp64_steps:.3f} ms/step, phases {ph64}")
-
+    # the FP64 form over the same window (warmup, then the same steps)
+    wall64, ph64, _ = time_gpu(sc, _lib.FP64, a.steps, a.warmup)
+    log(f"FP64: {wall64 * 1e3 / a.steps:.3f} ms/step, phases {ph64}")
     value = nf * a.steps / wall32
-    value64 = nf * a.fp64_steps / wall64
+    value64 = nf * a.steps / wall64
+    # both forms over the configured run's length (the workload's 1,000 steps)
+    longrun = None
+    if a.long_steps > 0 and a.long_steps != a.steps:
+        l32, _, _ = time_gpu(sc, _lib.FP32_BULK, a.long_steps, a.warmup, phase_steps=0)
+        l64, _, _ = time_gpu(sc, _lib.FP64, a.long_steps, a.warmup, phase_steps=0)
+        longrun = {"steps": a.long_steps, "warmup": a.warmup, "fp32_bulk_ms_per_step": l32 * 1e3 / a.long_steps,
+                   "fp64_ms_per_step": l64 * 1e3 / a.long_steps, "fp32_over_fp64": l64 / l32,
+                   "fp32_bulk_particle_steps_per_s": nf * a.long_steps / l32}
+        log(f"{a.long_steps}-step run: {longrun}")
     fb, wb = FORCE_BYTES[1]
     t_force = ph32["forces"] * 1e-3
     force_bytes = fb * nf + wb * nw
@@ -243,11 +282,13 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"CPU baseline: FP64 oracle, {a.cpu_steps} steps of the full scene on {threads} threads")
+        threads, host = host_cores()
+        log(f"CPU baseline: FP64 oracle, {a.cpu_steps} steps of the full scene on {threads} threads, {host}")
         v, dt = cpu_baseline(sc, a.cpu_steps, threads)
         cpu = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{a.cpu_steps} steps of the full {nf}-particle scene, FP64 oracle (OpenMP), {dt:.1f} s"}
+               "sample": f"{a.cpu_steps} steps of the full {nf}-particle scene (after one untimed step), "
+                         f"FP64 oracle (C, OpenMP), {dt:.1f} s",
+               **host}
 
     ens = None
     if not a.no_ensemble:
@@ -258,7 +299,7 @@ def main():
             ens = {"error": repr(e)}
         log(f"ensemble: {ens}")
     srv = None
-    if a.serving:
+    if not a.no_serving:
         log(f"serving: {a.serving_requests} requests")
         try:
             srv = serving_leg(a.serving_requests, a.serving_rate, a.serving_step_scale, 200.0, 50)
@@ -287,8 +328,10 @@ def main():
                      "algorithmic_bytes": force_bytes},
         "step_algorithmic_GBps": step_gbs,
         "phases_ms": {"fp32_bulk": ph32, "fp64": ph64},
-        "fp64_form": {"value": value64, "ms_per_step": wall64 * 1e3 / a.fp64_steps, "steps": a.fp64_steps},
+        "fp64_form": {"value": value64, "ms_per_step": wall64 * 1e3 / a.steps, "steps": a.steps,
+                      "warmup": a.warmup},
         "fp32_over_fp64": value / value64,
+        "long_run": longrun,
         "cpu_baseline": cpu,
         "ensemble": ens,
         "serving": srv,

@@ -961,8 +961,10 @@ extern "C" int esph_scene_group_count(esph_scene* s, int32_t group, int64_t* n) 
     return ESPH_OK;
 }
 
-// Pair export through k_pairs: count, host scan, write, remap, sort.
-static int export_pairs(esph_scene* s, int pairset, std::vector<int64_t>& pairs_out) {
+// Pair export through k_pairs (cell scan, binary64 rule) or, with walk, through
+// k_walk_pairs (the pair passes' own decisions): count, host scan, write,
+// remap, sort.
+static int export_pairs(esph_scene* s, int pairset, std::vector<int64_t>& pairs_out, bool walk = false) {
     esph_ctx* ctx = s->ctx;
     SceneDyn dyn;
     int rc = read_dyn(s, &dyn);
@@ -974,7 +976,10 @@ static int export_pairs(esph_scene* s, int pairset, std::vector<int64_t>& pairs_
     int64_t* offs = (int64_t*)a.get(sizeof(int64_t) * std::max(n, 1));
     if (!counts || !offs) { a.release(); return fail(ctx, ESPH_ERR_HIP, "export: allocation failed"); }
     const Bucket b = scene_bucket(s);
-    hipError_t e = launch_pairs(b, pairset, counts, nullptr, nullptr, st);
+    auto launch = [&](int32_t* c, const int64_t* o, int32_t* out) {
+        return walk ? launch_walk_pairs(b, pairset, c, o, out, st) : launch_pairs(b, pairset, c, o, out, st);
+    };
+    hipError_t e = launch(counts, nullptr, nullptr);
     std::vector<int32_t> hc(n);
     if (e == hipSuccess && n) e = hipMemcpyAsync(hc.data(), counts, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -986,7 +991,7 @@ static int export_pairs(esph_scene* s, int pairset, std::vector<int64_t>& pairs_
     if (!dout) { a.release(); return fail(ctx, ESPH_ERR_HIP, "export: allocation failed"); }
     std::vector<int32_t> hp(2 * m);
     if (n) e = hipMemcpyAsync(offs, ho.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = launch_pairs(b, pairset, nullptr, offs, dout, st);
+    if (e == hipSuccess) e = launch(nullptr, offs, dout);
     if (e == hipSuccess && m) e = hipMemcpyAsync(hp.data(), dout, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     a.release();
@@ -1018,6 +1023,26 @@ extern "C" int esph_scene_neighbours(esph_scene* s, int32_t pairset, int32_t* ou
     *m = (int64_t)pairs.size();
     if (!out_pairs && capacity == 0) return ESPH_OK;
     if ((int64_t)pairs.size() > capacity) return fail(s->ctx, ESPH_ERR_CAPACITY, "neighbours: output too small");
+    for (size_t q = 0; q < pairs.size(); ++q) {
+        out_pairs[2 * q] = (int32_t)(pairs[q] >> 32);
+        out_pairs[2 * q + 1] = (int32_t)(pairs[q] & 0xffffffff);
+    }
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_walk_pairs(esph_scene* s, int32_t pairset, int32_t* out_pairs, int64_t capacity,
+                                     int64_t* m) {
+    if (!s || !m || (pairset != ESPH_PAIRS_FLUID_FLUID && pairset != ESPH_PAIRS_FLUID_WALL))
+        return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "walk_pairs: bad argument");
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    if (s->h.flags & FLAG_IO) return fail(s->ctx, ESPH_ERR_ARG, "walk_pairs: not for the inlet/outlet solver");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    std::vector<int64_t> pairs;
+    int rc = export_pairs(s, pairset, pairs, true);
+    if (rc) return rc;
+    *m = (int64_t)pairs.size();
+    if (!out_pairs && capacity == 0) return ESPH_OK;
+    if ((int64_t)pairs.size() > capacity) return fail(s->ctx, ESPH_ERR_CAPACITY, "walk_pairs: output too small");
     for (size_t q = 0; q < pairs.size(); ++q) {
         out_pairs[2 * q] = (int32_t)(pairs[q] >> 32);
         out_pairs[2 * q + 1] = (int32_t)(pairs[q] & 0xffffffff);

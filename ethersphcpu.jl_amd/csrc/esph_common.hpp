@@ -220,6 +220,19 @@ __device__ __forceinline__ int build_list(const SceneDev& s, const int32_t* __re
     return cnt;
 }
 
+// Rebuilds particle i's fluid and wall candidate lists from the cells (right
+// after a sort, or every step when the lists are not reused) and returns the
+// fluid candidate count.
+template <int D>
+__device__ __forceinline__ int rebuild_lists(const SceneDev& s, const FluidSet& S, const double* xi, int i) {
+    int c[3];
+    cell_of<D>(s, xi, c);
+    const int cnt = build_list<D>(s, s.cell_start, S.xf, S.x, xi, c, i, i, s.vnbr, s.kff);
+    s.nvnb[i] = cnt;
+    if (s.nw > 0) s.nnbw[i] = build_list<D>(s, s.wcell_start, s.wxf, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
+    return cnt;
+}
+
 // Iterates the recorded neighbours of particle i with the same (j, d2, rv)
 // arguments as for_each_neighbour; the next entry's index and position are
 // loaded while the current one is processed.

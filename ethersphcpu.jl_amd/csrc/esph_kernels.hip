@@ -511,6 +511,78 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const SceneDev* __restrict__ s
     }
 }
 
+// The pair decisions of the step's own walks, exported for checking: for
+// each live fluid particle i, every candidate of its list that the pair
+// passes weight with 1 -- the quantized-geometry decision of the FP32_BULK
+// walks (qgeom, taken by the continuity pass and the 3D force pass), the
+// binary64 rule over the candidate list in FP64, and for walls the binary64
+// rule over the wall candidates; particles whose lists overflowed take the
+// cell scan of the SCAN instances.  Directed pairs (id_i, id_j): both orders
+// of a fluid-fluid pair appear when both walks decide it.  Lists that the
+// next continuity pass would rebuild are rebuilt here the same way first.
+template <int D, typename T>
+__global__ __launch_bounds__(kBlock) void k_walk_pairs(const SceneDev* __restrict__ scenes,
+                                                       const Tile* __restrict__ tiles, int ntiles, int pairset,
+                                                       int32_t* __restrict__ counts, const int64_t* __restrict__ offs,
+                                                       int32_t* __restrict__ out) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
+        const Tile tile = tiles[t];
+        const SceneDev& s = scenes[tile.scene];
+        const SceneDyn dyn = *s.dyn;
+        const int i = tile.begin + (int)threadIdx.x;
+        if (i >= dyn.n_live) continue;
+        const FluidSet& S = s.set[cur_set(dyn)];
+        double xi[3];
+        load_x<D>(S.x, i, xi);
+        if (dyn.rebuild || !s.verlet) rebuild_lists<D>(s, S, xi, i);
+        int64_t w = offs ? offs[i] : 0;
+        int cnt = 0;
+        const int idi = S.id[i];
+        auto emit = [&](int idj) {
+            if (out) {
+                out[2 * w] = idi;
+                out[2 * w + 1] = idj;
+                ++w;
+            }
+            ++cnt;
+        };
+        const double R2 = s.R2;
+        const bool scan = overflowed(s, i);
+        if (pairset == ESPH_PAIRS_FLUID_FLUID) {
+            if (scan) {
+                int c[3];
+                cell_of<D>(s, xi, c);
+                for_each_neighbour<D>(s, s.cell_start, S.x, xi, c, i,
+                                      [&](int j, double, const double*) { emit(S.id[j]); });
+            } else if constexpr (sizeof(T) == 4) {
+                for_pairs_q<D, 1>(s, S, xi, i, s.vnbr, s.nvnb[i], [](int) { return 0; },
+                                  [&](int j, bool in, float, const float*, int) {
+                                      if (in) emit(S.id[j]);
+                                  });
+            } else {
+                for_pairs_d<D, false, 1>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], [](int) { return 0; },
+                                         [&](int j, double d2, const double*, int) {
+                                             if (d2 <= R2) emit(S.id[j]);
+                                         });
+            }
+        } else if (s.nw > 0) {
+            if (scan) {
+                int c[3];
+                cell_of<D>(s, xi, c);
+                for_each_neighbour<D>(s, s.wcell_start, s.wx, xi, c, -1,
+                                      [&](int j, double, const double*) { emit(s.wid[j]); });
+            } else {
+                for_pairs_d<D, false, 1>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], [](int) { return 0; },
+                                         [&](int j, double d2, const double*, int) {
+                                             if (d2 <= R2) emit(s.wid[j]);
+                                         });
+            }
+        }
+        if (counts) counts[i] = cnt;
+    }
+}
+
 // kernelValueInterpolation (DataIO/VTPToDataFrame.jl:77-142) over the
 // resident fluid: one probe point per thread, the probe's 3^D cells of the
 // current cell list, FP64 sums of W m_j / rho_j and W m_j / rho_j f_j in cell
@@ -592,6 +664,21 @@ hipError_t launch_pairs(const Bucket& b, int pairset, int32_t* counts, const int
         hipLaunchKernelGGL(k_pairs<2>, g, blk, 0, st, b.scenes, b.tiles, b.ntiles, pairset, counts, offs, out);
     else
         hipLaunchKernelGGL(k_pairs<3>, g, blk, 0, st, b.scenes, b.tiles, b.ntiles, pairset, counts, offs, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk_pairs(const Bucket& b, int pairset, int32_t* counts, const int64_t* offs, int32_t* out,
+                             hipStream_t st) {
+    if (b.ntiles <= 0) return hipSuccess;
+    dim3 g(grid_for(b.ntiles)), blk(kBlock);
+    const bool f32 = b.prec == ESPH_FP32_BULK;
+    if (b.dim == 2) {
+        if (f32) hipLaunchKernelGGL((k_walk_pairs<2, float>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles, pairset, counts, offs, out);
+        else hipLaunchKernelGGL((k_walk_pairs<2, double>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles, pairset, counts, offs, out);
+    } else {
+        if (f32) hipLaunchKernelGGL((k_walk_pairs<3, float>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles, pairset, counts, offs, out);
+        else hipLaunchKernelGGL((k_walk_pairs<3, double>), g, blk, 0, st, b.scenes, b.tiles, b.ntiles, pairset, counts, offs, out);
+    }
     return hipGetLastError();
 }
 

@@ -46,6 +46,8 @@ hipError_t launch_shepard(const Bucket& b, hipStream_t st);
 hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st);
 hipError_t launch_pairs(const Bucket& b, int pairset, int32_t* counts, const int64_t* offs, int32_t* out,
                         hipStream_t st);
+hipError_t launch_walk_pairs(const Bucket& b, int pairset, int32_t* counts, const int64_t* offs, int32_t* out,
+                             hipStream_t st);
 hipError_t launch_kernel_eval(int kernel, int64_t n, const double* r, double* w, double* dw, double h,
                               double sigma, hipStream_t st);
 hipError_t launch_probe(const Bucket& b, int64_t np, const double* pts, int field, double none_value, double* out,

@@ -48,11 +48,7 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
         if (!SCAN) {
             if (dyn.rebuild || !s.verlet) {
                 // right after a sort: candidate lists within Rv from the cells
-                int c[3];
-                cell_of<D>(s, xi, c);
-                cnt = build_list<D>(s, s.cell_start, S.xf, S.x, xi, c, i, i, s.vnbr, s.kff);
-                s.nvnb[i] = cnt;
-                if (s.nw > 0) s.nnbw[i] = build_list<D>(s, s.wcell_start, s.wxf, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
+                cnt = rebuild_lists<D>(s, S, xi, i);
             } else {
                 cnt = s.nvnb[i];
             }

@@ -294,6 +294,17 @@ class Scene:
                                                     out.shape[0], byref(m)))
         return out[: m.value]
 
+    def walk_pairs(self, pairset: int = _lib.PAIRS_FLUID_FLUID) -> np.ndarray:
+        """Sorted (m, 2) int32 directed pairs that the next step's pair
+        passes decide to be neighbours (esph_scene_walk_pairs)."""
+        m = c_int64()
+        self.ctx.check(self.L.esph_scene_walk_pairs(self.handle, pairset, None, 0, byref(m)))
+        out = np.empty((max(m.value, 1), 2), dtype=np.int32)
+        self.ctx.check(self.L.esph_scene_walk_pairs(self.handle, pairset,
+                                                    out.ctypes.data_as(ctypes.POINTER(c_int32)),
+                                                    out.shape[0], byref(m)))
+        return out[: m.value]
+
     def probe(self, points, field: int, none_value: float = float("nan")) -> np.ndarray:
         """SPH interpolation of a fluid field at points (n, dim):
         kernelValueInterpolation (DataIO/VTPToDataFrame.jl:77-142) on the

@@ -309,6 +309,19 @@ int esph_scene_get_group(esph_scene* scene, int32_t group, int32_t field, double
    sorted; same output contract as esph_neighbor_list. */
 int esph_scene_neighbours(esph_scene* scene, int32_t pairset, int32_t* out_pairs,
                           int64_t capacity, int64_t* m);
+/* The pair decisions the next step's pair passes will take themselves
+   (findNeighbours!, Neighbour/Neighbour.jl:88-128, as this library makes
+   it): every candidate of each live fluid particle's list that the passes
+   weight with 1 -- in ESPH_FP32_BULK the quantized-geometry decision of the
+   continuity and (3D) force walks, in ESPH_FP64 the binary64 rule over the
+   candidate list; wall candidates always by the binary64 rule; particles
+   whose lists overflowed by the cell scan.  Directed pairs: ESPH_PAIRS_FLUID_FLUID
+   lists (i, j) and (j, i) when both walks take the pair, so a pair set
+   following the neighbour rule gives each unordered pair of
+   esph_scene_neighbours twice.  Same output contract as esph_neighbor_list;
+   ESPH_ERR_ARG for the inlet/outlet solver. */
+int esph_scene_walk_pairs(esph_scene* scene, int32_t pairset, int32_t* out_pairs,
+                          int64_t capacity, int64_t* m);
 /* kernelValueInterpolation (DataIO/VTPToDataFrame.jl:77-142) on the resident
    fluid, FP64: at each of npoints probe points (npoints*dim doubles),
    sum_j W(r) m_j / rho_j f_j / sum_j W(r) m_j / rho_j over the fluid particles

@@ -39,10 +39,8 @@ def test_dam_break_3d_full_size_1000_steps():
     a = fields(sc, _lib.FP32_BULK, 1000)
     b = fields(sc, _lib.FP64, 1000)
     errs = {k: rel_l2(a[k], b[k]) for k in a}
-    # p is measured against its range: p = c0^2 (rho - rho0) is a small difference of large numbers
     say("3D dam break FP32_BULK vs FP64 after 1000 steps, rel-L2:", {k: f"{v:.2e}" for k, v in errs.items()})
-    assert all(v < 1e-4 for k, v in errs.items() if k != "p"), errs
-    assert errs["p"] < 1e-3, errs
+    assert all(v < 1e-4 for v in errs.values()), errs
 
 
 def test_thermal_cavity_40k_drift():

@@ -232,29 +232,40 @@ def test_fp32_bulk_drift_dam_break_2d(ctx):
 
 @pytest.mark.parametrize("dim,n", [(2, 40), (3, 14)])
 def test_fp32_bulk_pairs_near_R_follow_fp64_rule(ctx, dim, n):
-    """Gaussian lattice with h = dx (1 + 1e-7) and random velocities: many
-    pairs lie 1e-7 (relative) inside R = 3 h -- inside the band where the
-    FP32_BULK continuity pass's quantized float distance cannot decide, so
-    the FP64 rule and binary64 geometry must.  Dropping those pairs would move
-    drho by 9e-4 (2D) / 2e-3 (3D) relative L2 (the oracle with R 2e-7 smaller); the
-    device's drho after one step stays within 5e-5 of the FP64 oracle's.
-    (Exactly at R the truncated Gaussian jumps to zero, and a float q = r/h
-    that rounds to 3 drops the pair from the sum whatever the rule says; the
-    lattice is therefore kept just inside R.)"""
+    """Gaussian lattice with h = dx (1 + 1e-7) and random velocities above a
+    3-layer wall plane: many fluid-fluid and fluid-wall pairs lie 1e-7
+    (relative) inside R = 3 h -- inside the band where the FP32_BULK walks'
+    quantized float distance cannot decide, so the FP64 rule and binary64
+    geometry must.  Dropping those pairs (the oracle with R 2e-7 smaller)
+    moves drho by 9e-4 (2D) / 2e-3 (3D) and dv by 1.1e-3 / 2.4e-3 relative
+    L2; after one step the device's drho and dv (continuity pass, and the
+    force pass's fluid and wall walks) stay within 5e-5 of the FP64 oracle's
+    and the positions' displacement within 1e-6.  The walks' own decisions
+    equal the oracle's pair sets.  (Exactly at R the truncated Gaussian jumps
+    to zero, and a float q = r/h that rounds to 3 drops the pair from the sum
+    whatever the rule says; the lattice is therefore kept just inside R.)"""
     sc = lattice_scene(n=n, dx=0.01, dim=dim, kernel="Gaussian", h_factor=1.0 + 1e-7)
     rng = np.random.default_rng(2)
     sc.fluid.v_vec_ = rng.uniform(-0.5, 0.5, sc.fluid.x_vec_.shape)
+    sc.walls = [wall_plane(dim, n, 0.01)]
     r0 = sc.fluid.rho_.copy()
+    v0 = sc.fluid.v_vec_.copy()
+    x0 = sc.fluid.x_vec_.copy()
     dev = sc.make_scene(_lib.FP32_BULK, ctx)
     ref = O.from_scenario(sc)
+    ff = ref.neighbours(_lib.PAIRS_FLUID_FLUID)
+    fw = ref.neighbours(_lib.PAIRS_FLUID_WALL)
+    w = dev.walk_pairs(_lib.PAIRS_FLUID_FLUID)
+    assert np.array_equal(w[w[:, 0] < w[:, 1]], ff) and len(w) == 2 * len(ff)
+    assert np.array_equal(dev.walk_pairs(_lib.PAIRS_FLUID_WALL), fw)
     dev.step(1)
     dev.sync()
     ref.step(1)
-    d_dev = dev.get(_lib.FIELD_RHO) - r0
-    d_ref = ref.get(_lib.FIELD_RHO) - r0
-    err = rel_l2(d_dev, d_ref)
-    say("fp32 pairs near R", dim, err)
-    assert err < 5e-5, err
+    errs = {"drho": rel_l2(dev.get(_lib.FIELD_RHO) - r0, ref.get(_lib.FIELD_RHO) - r0),
+            "dv": rel_l2(dev.get(_lib.FIELD_V) - v0, ref.get(_lib.FIELD_V) - v0),
+            "dx": rel_l2(dev.get(_lib.FIELD_X) - x0, ref.get(_lib.FIELD_X) - x0)}
+    say("fp32 pairs near R", dim, errs)
+    assert errs["drho"] < 5e-5 and errs["dv"] < 5e-5 and errs["dx"] < 1e-6, errs
 
 
 def test_check_deletion_matches_oracle(ctx):
