@@ -16,7 +16,7 @@ namespace esph {
 // ---------------------------------------------------------------------------
 // Counting sort of the step's new positions into cells.  Deterministic: the
 // order inside a cell is the particles' previous order.  The check predicate
-// (FluidWallSolve.jl:89-97, deleteat! is stable) sends deleted particles to
+// (FluidWallSolve.jl:73-81, deleteat! is stable) sends deleted particles to
 // the tail bucket `ncell`, which is dropped.
 // mode 0: end of a step; mode 1: upload (rho from rhon, no check).
 // The cell counts are zero on entry: the allocation clears them and the

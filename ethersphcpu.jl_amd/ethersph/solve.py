@@ -2,8 +2,8 @@
 
 ``solve(fluid, walls..., smooth_kernel, model, time_discretization, ...)``
 picks the solver the way the reference's multiple dispatch does
-(Solve/FluidWallSolve.jl:101, FluidVelocitySolve.jl:104,
-FluidThermostaticWallSolve.jl:127, FluidCompulsiveThermostaticWallSolve.jl:154),
+(Solve/FluidWallSolve.jl:85-140, FluidVelocitySolve.jl:104-168,
+FluidThermostaticWallSolve.jl:127-173, FluidCompulsiveThermostaticWallSolve.jl:154-233),
 uploads the particles once, advances ``total_step_`` steps on the GPU and
 writes the state back into the caller's particles (deleting, in order, the
 particles the check removed, as ``deleteat!`` does).
@@ -60,7 +60,7 @@ class AABBCheck:
 
 
 class InletOutletPlanes:
-    """The inlet/outlet closures of FluidInletOutletWallSolve.jl:176-180 as
+    """The inlet/outlet closures of FluidInletOutletWallSolve.jl:275-279 as
     data, in the form the Poiseuille examples use (PoiseuilleFlow2D.jl:146-151):
     along ``axis``, an inlet particle past ``inlet_enter`` is copied into the
     fluid and moved back by ``inlet_shift``; a fluid particle past
@@ -100,7 +100,7 @@ def select_solver(walls: list, model, buffers: bool = False) -> int:
     """Dispatch of solve! on particle-vector and model types."""
     thermal = isinstance(model, ThermalWeaklyCompressibleLiquidModel)
     kinds = [w.kind for w in walls]
-    if buffers:  # FluidInletOutletWallSolve.jl:167-181
+    if buffers:  # FluidInletOutletWallSolve.jl:266-340
         if len(walls) == 1 and not thermal:
             return _lib.SOLVER_FLUID_INLET_OUTLET_WALL
         raise TypeError("the inlet/outlet solver takes one wall group and a common liquid model")
@@ -391,7 +391,7 @@ def solve(*args, check=None, precision: str = "FP64", ctx=None, on_output=None, 
     time_discretization[, vtp_io]; check), and the inlet/outlet form
     solve!(fluid, inlet, outlet, walls, kernel, model, td[, vtp_io];
     inlet_outlet=InletOutletPlanes, user_defined_function=BufferProfile)
-    (FluidInletOutletWallSolve.jl:167-181).
+    (FluidInletOutletWallSolve.jl:266-340).
 
     Particle groups come first; the first SmoothKernel starts the trailing
     (kernel, model, time_discretization[, vtp_io]) arguments.  ``check`` must

@@ -8,11 +8,11 @@
   the C ABI expects), handed over once, stepped on the GPU and unpacked.
 
   Interfaces replaced (EtherSPHCPU.jl):
-    solve!(fluid, wall, kernel, model, td, vtp_io; check)            Solve/FluidWallSolve.jl:101
-    solve!(fluid, static_velocity_wall, ...)                          Solve/FluidVelocitySolve.jl:104
-    solve!(fluid, thermostatic_wall, ..., ::ThermalWeakly...Model)    Solve/FluidThermostaticWallSolve.jl:127
-    solve!(fluid, wall, thermostatic_wall, ..., ::ThermalWeakly...)   Solve/FluidCompulsiveThermostaticWallSolve.jl:154
-    solve!(fluid, inlet, outlet, wall, ...; inlet/outlet closures)     Solve/FluidInletOutletWallSolve.jl:167
+    solve!(fluid, wall, kernel, model, td, vtp_io; check)            Solve/FluidWallSolve.jl:85-140
+    solve!(fluid, static_velocity_wall, ...)                          Solve/FluidVelocitySolve.jl:104-168
+    solve!(fluid, thermostatic_wall, ..., ::ThermalWeakly...Model)    Solve/FluidThermostaticWallSolve.jl:127-173
+    solve!(fluid, wall, thermostatic_wall, ..., ::ThermalWeakly...)   Solve/FluidCompulsiveThermostaticWallSolve.jl:154-233
+    solve!(fluid, inlet, outlet, wall, ...; inlet/outlet closures)     Solve/FluidInletOutletWallSolve.jl:266-340
     kernelValue / kernelGradient                                      SPHKernel/SmoothKernel/*.jl
     CellListMap.neighborlist (as used by findNeighbours!)             Neighbour/Neighbour.jl:88
   `check` closures cannot run on the device; pass an AABBCheck instead.  The
@@ -32,13 +32,20 @@ struct MI355X
 end
 MI355X() = MI355X(0)
 
-"The `check` predicate as data: keep iff lo <= x <= hi on every axis."
+"""The `check` predicate as data: keep iff lo <= x <= hi on every axis and
+n . x <= d for every plane (n, d) (at most 8; a convex keep-region)."""
 struct AABBCheck
     lo::Vector{Float64}
     hi::Vector{Float64}
+    planes::Vector{Tuple{Vector{Float64},Float64}}
 end
+AABBCheck(lo, hi) = AABBCheck(lo, hi, Tuple{Vector{Float64},Float64}[])
 
-"The inlet/outlet closures of FluidInletOutletWallSolve.jl:176-180 as planes along `axis` (1-based)."
+"Arithmetic form: `:fp64` (the reference's) or `:fp32_bulk` (binary32 pair terms, binary64 integration)."
+precision_code(p::Symbol) = p === :fp64 ? Int32(0) : p === :fp32_bulk ? Int32(1) :
+    throw(ArgumentError("precision must be :fp64 or :fp32_bulk"))
+
+"The inlet/outlet closures of FluidInletOutletWallSolve.jl:275-279 as planes along `axis` (1-based)."
 struct InletOutletPlanes
     axis::Int
     inlet_enter::Float64     # inlet particle with x[axis] > inlet_enter joins the fluid...
@@ -162,14 +169,24 @@ context(dev::MI355X) = get!(() -> GPUContext(dev.device), CONTEXTS, dev.device)
 packvec(ps, f) = reduce(hcat, [getfield(p, f) for p in ps]; init = zeros(Float64, length(getfield(first(ps), f)), 0))
 packsc(ps, f) = Float64[getfield(p, f) for p in ps]
 
-function descriptor(dim, kernel, model, td, solver, check; planes = nothing, profile = nothing, capacity = 0)
+function descriptor(dim, kernel, model, td, solver, check; planes = nothing, profile = nothing, capacity = 0,
+                    precision::Symbol = :fp64)
     bf = ntuple(k -> k <= dim ? Float64(model.body_force_vec_[k]) : 0.0, 3)
     thermal = model isa ThermalWeaklyCompressibleLiquidModel
     lo = ntuple(k -> (check !== nothing && k <= dim) ? check.lo[k] : -Inf, 3)
     hi = ntuple(k -> (check !== nothing && k <= dim) ? check.hi[k] : Inf, 3)
     pl = planes === nothing ? InletOutletPlanes(1, 0.0, 0.0, 0.0, 0.0) : planes
     pr = profile === nothing ? BufferProfile(0, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0), 0, false) : profile
-    return EsphSceneDesc(dim, kernel_kind(kernel), kernel.influence_radius_, solver, 0,
+    cpl = check === nothing ? Tuple{Vector{Float64},Float64}[] : check.planes
+    length(cpl) <= 8 || throw(ArgumentError("at most 8 check planes"))
+    flat = zeros(32)
+    for (q, (nrm, d)) in enumerate(cpl)
+        for k in 1:dim
+            flat[4 * (q - 1) + k] = nrm[k]
+        end
+        flat[4 * (q - 1) + 4] = d
+    end
+    return EsphSceneDesc(dim, kernel_kind(kernel), kernel.influence_radius_, solver, precision_code(precision),
                          model.rho_0_, model.c_0_, model.p_0_, model.gamma_, model.mu_0_, bf,
                          thermal ? model.t_0_ : 0.0, thermal ? model.kappa_0_ : 0.0,
                          thermal ? model.cp_ : 0.0, thermal ? model.beta_ : 0.0,
@@ -177,7 +194,7 @@ function descriptor(dim, kernel, model, td, solver, check; planes = nothing, pro
                          check === nothing ? 0 : 1, lo, hi, capacity, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0),
                          pl.axis - 1, pl.inlet_enter, pl.inlet_shift, pl.outlet_enter, pl.outlet_exit,
                          pr.kind, pr.terms, pr.outlet_profile ? 1 : 0, pr.f, pr.nu, pr.l, pr.center, 0.0,
-                         0, ntuple(_ -> 0.0, 32))
+                         length(cpl), ntuple(k -> flat[k], 32))
 end
 
 fluid_arrays(n, x, v, rho, p, c, m, gap) =
@@ -187,15 +204,55 @@ fluid_arrays(n, x, v, rho, p, c, m, gap) =
 pack_liquid(ps) = (packvec(ps, :x_vec_), packvec(ps, :v_vec_), packsc(ps, :rho_), packsc(ps, :p_), packsc(ps, :c_),
                    packsc(ps, :mass_), packsc(ps, :gap_))
 
-function run_gpu!(fluid, walls, kernel, model, td, solver, check, dev::MI355X)
+# Reads the device state back into the caller's fluid vector in reference
+# order (deleteat! of the particles the check removed, order kept).
+function unpack_fluid!(ctx, scene, fluid, dim, thermal)
+    n = Ref{Int64}(0)
+    chk(ctx, ccall((:esph_scene_fluid_count, libesph), Cint, (Ptr{Cvoid}, Ptr{Int64}), scene, n))
+    get(field, w) = begin
+        out = Vector{Float64}(undef, max(n[] * w, 1))
+        wr = Ref{Int64}(0)
+        chk(ctx, ccall((:esph_scene_get_fluid, libesph), Cint,
+                         (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Ptr{Int64}), scene, field, out, length(out), wr))
+        out[1:wr[]]
+    end
+    ids = Int.(get(10, 1)) .+ 1          # upload indices of the live particles
+    X = reshape(get(0, dim), dim, :); V = reshape(get(1, dim), dim, :)
+    R = get(2, 1); P = get(3, 1); C = get(4, 1)
+    T = thermal ? get(7, 1) : Float64[]
+    return ids, X, V, R, P, C, T
+end
+
+# solve! of Solve/FluidWallSolve.jl:85-140 (and the other fixed-count
+# solvers) on the device: upload once, writeVTP at step 0, then step to each
+# output step (isOutputStep, FixedTimeIntervalDiscretization.jl:18-20) and
+# write frame div(step, output_interval_) there with the reference's own
+# writeVTP (:95, :136).  A failed chunk of steps (a non-finite position met on
+# the device) takes the reference's catch path (:130-134): the error and the
+# step are printed, the state is dumped with writeVTP(step, ...) and stepping
+# continues.  The device reports such an error for a chunk, so the printed
+# step is the last step of the chunk it was met in.
+function run_gpu!(fluid, walls, kernel, model, td, solver, check, vtp_io, dev::MI355X; precision::Symbol = :fp64)
     ctx = context(dev)
     dim = length(first(fluid).x_vec_)
-    desc = Ref(descriptor(dim, kernel, model, td, solver, check))
+    desc = Ref(descriptor(dim, kernel, model, td, solver, check; precision = precision))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     chk(ctx, ccall((:esph_scene_create, libesph), Cint, (Ptr{Cvoid}, Ptr{EsphSceneDesc}, Ptr{Ptr{Cvoid}}),
                      ctx.handle, desc, h))
     scene = h[]
     thermal = first(fluid) isa ThermalLiquidParticle && model isa ThermalWeaklyCompressibleLiquidModel
+    upload = copy(fluid)                  # the particle objects in upload order
+    groups() = Any[fluid, walls...]
+    function write_back!()
+        ids, X, V, R, P, C, T = unpack_fluid!(ctx, scene, fluid, dim, thermal)
+        keep = upload[ids]
+        resize!(fluid, length(keep)); fluid .= keep
+        for (i, q) in enumerate(fluid)
+            q.x_vec_ .= X[:, i]; q.v_vec_ .= V[:, i]
+            q.rho_ = R[i]; q.p_ = P[i]; q.c_ = C[i]
+            thermal && (q.t_ = T[i])
+        end
+    end
     try
         # walls: all groups at once
         wpacked = [(wall_kind(w), length(w), packvec(w, :x_vec_), packvec(w, :normal_vec_), packsc(w, :gap_),
@@ -221,28 +278,30 @@ function run_gpu!(fluid, walls, kernel, model, td, solver, check, dev::MI355X)
                                      thermal ? pointer(kap) : C_NULL, thermal ? pointer(cp) : C_NULL))
             chk(ctx, ccall((:esph_scene_set_fluid, libesph), Cint, (Ptr{Cvoid}, Ptr{EsphFluidArrays}), scene, fa))
         end
-        chk(ctx, ccall((:esph_scene_step, libesph), Cint, (Ptr{Cvoid}, Int64), scene, td.total_step_))
-        chk(ctx, ccall((:esph_scene_sync, libesph), Cint, (Ptr{Cvoid},), scene))
-        n = Ref{Int64}(0)
-        chk(ctx, ccall((:esph_scene_fluid_count, libesph), Cint, (Ptr{Cvoid}, Ptr{Int64}), scene, n))
-        get(field, w) = begin
-            out = Vector{Float64}(undef, n[] * w)
-            wr = Ref{Int64}(0)
-            chk(ctx, ccall((:esph_scene_get_fluid, libesph), Cint,
-                             (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Ptr{Int64}), scene, field, out, length(out), wr))
-            out
+        assureDirPathExist(vtp_io)
+        writeVTP(0, 0.0, vtp_io, groups())
+        total = td.total_step_
+        every = td.output_interval_ > 0 ? td.output_interval_ : total
+        step = 0
+        while step < total
+            next = min(total, (div(step, every) + 1) * every)
+            rc = ccall((:esph_scene_step, libesph), Cint, (Ptr{Cvoid}, Int64), scene, next - step)
+            rc == 0 && (rc = ccall((:esph_scene_sync, libesph), Cint, (Ptr{Cvoid},), scene))
+            step = next
+            if rc != 0
+                println(ErrorException("ethersph error $rc: " *
+                                       unsafe_string(ccall((:esph_last_error, libesph), Cstring, (Ptr{Cvoid},),
+                                                           ctx.handle))))
+                println("Error at step: ", step)
+                write_back!()
+                writeVTP(step, step * td.dt_, vtp_io, groups())
+            end
+            if isOutputStep(step, td)
+                write_back!()
+                writeVTP(div(step, td.output_interval_), step * td.dt_, vtp_io, groups())
+            end
         end
-        ids = Int.(get(10, 1)) .+ 1
-        keep = fluid[ids]
-        resize!(fluid, length(keep)); fluid .= keep          # deleteat! of the removed ones, order kept
-        X = reshape(get(0, dim), dim, :); V = reshape(get(1, dim), dim, :)
-        R = get(2, 1); P = get(3, 1); C = get(4, 1)
-        T = thermal ? get(7, 1) : Float64[]
-        for (i, q) in enumerate(fluid)
-            q.x_vec_ .= X[:, i]; q.v_vec_ .= V[:, i]
-            q.rho_ = R[i]; q.p_ = P[i]; q.c_ = C[i]
-            thermal && (q.t_ = T[i])
-        end
+        write_back!()
     finally
         ccall((:esph_scene_destroy, libesph), Cint, (Ptr{Cvoid},), scene)
     end
@@ -253,48 +312,51 @@ end
 function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:WallParticle},
                             kernel::SmoothKernel, model::WeaklyCompressibleLiquidModel,
                             td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
-                            check::Union{Nothing,AABBCheck} = nothing)
-    run_gpu!(fluid, [wall], kernel, model, td, Int32(0), check, dev)
+                            check::Union{Nothing,AABBCheck} = nothing, precision::Symbol = :fp64)
+    run_gpu!(fluid, [wall], kernel, model, td, Int32(0), check, vtp_io, dev; precision = precision)
 end
 
 function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:StaticVelocityWallParticle},
                             kernel::SmoothKernel, model::WeaklyCompressibleLiquidModel,
                             td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
-                            check::Union{Nothing,AABBCheck} = nothing)
-    run_gpu!(fluid, [wall], kernel, model, td, Int32(1), check, dev)
+                            check::Union{Nothing,AABBCheck} = nothing, precision::Symbol = :fp64)
+    run_gpu!(fluid, [wall], kernel, model, td, Int32(1), check, vtp_io, dev; precision = precision)
 end
 
 function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:ThermostaticWallParticle},
                             kernel::SmoothKernel, model::ThermalWeaklyCompressibleLiquidModel,
                             td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
-                            check::Union{Nothing,AABBCheck} = nothing)
-    run_gpu!(fluid, [wall], kernel, model, td, Int32(2), check, dev)
+                            check::Union{Nothing,AABBCheck} = nothing, precision::Symbol = :fp64)
+    run_gpu!(fluid, [wall], kernel, model, td, Int32(2), check, vtp_io, dev; precision = precision)
 end
 
 function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:WallParticle},
                             thermo::AbstractVector{<:ThermostaticWallParticle}, kernel::SmoothKernel,
                             model::ThermalWeaklyCompressibleLiquidModel,
                             td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
-                            check::Union{Nothing,AABBCheck} = nothing)
-    run_gpu!(fluid, [wall, thermo], kernel, model, td, Int32(3), check, dev)
+                            check::Union{Nothing,AABBCheck} = nothing, precision::Symbol = :fp64)
+    run_gpu!(fluid, [wall, thermo], kernel, model, td, Int32(3), check, vtp_io, dev; precision = precision)
 end
 
-# FluidInletOutletWallSolve.jl:167-181: the groups change size (inlet copies join the
+# FluidInletOutletWallSolve.jl:266-340: the groups change size (inlet copies join the
 # fluid, leavers join the outlet, expired outlet particles are removed); the
 # caller's vectors are refilled in reference order.
 function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, inlet::AbstractVector{<:FluidParticle},
                             outlet::AbstractVector{<:FluidParticle}, wall::AbstractVector{<:WallParticle},
                             kernel::SmoothKernel, model::WeaklyCompressibleLiquidModel,
                             td::DensityReinitializedFixedTimeIntervalForwardEuler, vtp_io, dev::MI355X;
-                            inlet_outlet::InletOutletPlanes, user_defined_function::Union{Nothing,BufferProfile} = nothing)
+                            inlet_outlet::InletOutletPlanes, user_defined_function::Union{Nothing,BufferProfile} = nothing,
+                            precision::Symbol = :fp64)
     ctx = context(dev)
     dim = length(first(fluid).x_vec_)
     desc = Ref(descriptor(dim, kernel, model, td, Int32(4), nothing; planes = inlet_outlet,
-                          profile = user_defined_function))
+                          profile = user_defined_function, precision = precision))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     chk(ctx, ccall((:esph_scene_create, libesph), Cint, (Ptr{Cvoid}, Ptr{EsphSceneDesc}, Ptr{Ptr{Cvoid}}),
                      ctx.handle, desc, h))
     scene = h[]
+    protos = (deepcopy(first(fluid)), deepcopy(isempty(inlet) ? first(fluid) : first(inlet)),
+              deepcopy(isempty(outlet) ? first(fluid) : first(outlet)))
     try
         wx = packvec(wall, :x_vec_); wn = packvec(wall, :normal_vec_); wg = packsc(wall, :gap_)
         wz = zeros(dim, length(wall)); wz1 = zeros(length(wall))
@@ -312,31 +374,51 @@ function EtherSPHCPU.solve!(fluid::AbstractVector{<:FluidParticle}, inlet::Abstr
             fa = Ref(fluid_arrays(length(fluid), fl...))
             chk(ctx, ccall((:esph_scene_set_fluid, libesph), Cint, (Ptr{Cvoid}, Ptr{EsphFluidArrays}), scene, fa))
         end
-        chk(ctx, ccall((:esph_scene_step, libesph), Cint, (Ptr{Cvoid}, Int64), scene, td.total_step_))
-        chk(ctx, ccall((:esph_scene_sync, libesph), Cint, (Ptr{Cvoid},), scene))
-        for (group, ps) in ((0, fluid), (1, inlet), (2, outlet))
-            n = Ref{Int64}(0)
-            chk(ctx, ccall((:esph_scene_group_count, libesph), Cint, (Ptr{Cvoid}, Int32, Ptr{Int64}), scene, group, n))
-            get(field, w) = begin
-                out = Vector{Float64}(undef, max(n[] * w, 1))
-                wr = Ref{Int64}(0)
-                chk(ctx, ccall((:esph_scene_get_group, libesph), Cint,
-                                 (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Int64, Ptr{Int64}),
-                                 scene, group, field, out, length(out), wr))
-                out[1:wr[]]
-            end
-            proto = first(ps)
-            while length(ps) < n[]
-                push!(ps, deepcopy(proto))
-            end
-            resize!(ps, n[])
-            X = reshape(get(0, dim), dim, :); V = reshape(get(1, dim), dim, :)
-            R = get(2, 1); P = get(3, 1); C = get(4, 1); M = get(5, 1); G = get(6, 1)
-            for (i, q) in enumerate(ps)
-                q.x_vec_ .= X[:, i]; q.v_vec_ .= V[:, i]
-                q.rho_ = R[i]; q.p_ = P[i]; q.c_ = C[i]; q.mass_ = M[i]; q.gap_ = G[i]
+        function write_back!()
+            for (group, ps) in ((0, fluid), (1, inlet), (2, outlet))
+                n = Ref{Int64}(0)
+                chk(ctx, ccall((:esph_scene_group_count, libesph), Cint, (Ptr{Cvoid}, Int32, Ptr{Int64}),
+                                 scene, group, n))
+                get(field, w) = begin
+                    out = Vector{Float64}(undef, max(n[] * w, 1))
+                    wr = Ref{Int64}(0)
+                    chk(ctx, ccall((:esph_scene_get_group, libesph), Cint,
+                                     (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Int64, Ptr{Int64}),
+                                     scene, group, field, out, length(out), wr))
+                    out[1:wr[]]
+                end
+                proto = protos[group + 1]
+                while length(ps) < n[]
+                    push!(ps, deepcopy(proto))
+                end
+                resize!(ps, n[])
+                X = reshape(get(0, dim), dim, :); V = reshape(get(1, dim), dim, :)
+                R = get(2, 1); P = get(3, 1); C = get(4, 1); M = get(5, 1); G = get(6, 1)
+                for (i, q) in enumerate(ps)
+                    q.x_vec_ .= X[:, i]; q.v_vec_ .= V[:, i]
+                    q.rho_ = R[i]; q.p_ = P[i]; q.c_ = C[i]; q.mass_ = M[i]; q.gap_ = G[i]
+                end
             end
         end
+        groups() = Any[fluid, inlet, outlet, wall]
+        # as the fixed-count solvers: frame 0, then every output step
+        # (FluidInletOutletWallSolve.jl:282,331)
+        assureDirPathExist(vtp_io)
+        writeVTP(0, 0.0, vtp_io, groups())
+        total = td.total_step_
+        every = td.output_interval_ > 0 ? td.output_interval_ : total
+        step = 0
+        while step < total
+            next = min(total, (div(step, every) + 1) * every)
+            chk(ctx, ccall((:esph_scene_step, libesph), Cint, (Ptr{Cvoid}, Int64), scene, next - step))
+            chk(ctx, ccall((:esph_scene_sync, libesph), Cint, (Ptr{Cvoid},), scene))
+            step = next
+            if isOutputStep(step, td)
+                write_back!()
+                writeVTP(div(step, td.output_interval_), step * td.dt_, vtp_io, groups())
+            end
+        end
+        write_back!()
     finally
         ccall((:esph_scene_destroy, libesph), Cint, (Ptr{Cvoid},), scene)
     end

@@ -964,7 +964,7 @@ static void step_once(eo_scene* s, int nthreads) {
         }
     }
     if (io) migrate(s);
-    /* check + deleteat! (FluidWallSolve.jl:89-97): stable removal */
+    /* check + deleteat! (FluidWallSolve.jl:73-81): stable removal */
     if (!io && s->d.check_enabled && s->d.check_interval > 0 && s->step % s->d.check_interval == 0) {
         int64_t w = 0;
         for (int64_t i = 0; i < s->n; ++i) {

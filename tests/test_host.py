@@ -141,3 +141,39 @@ def test_product_has_no_oracle_dependency():
         if f.endswith((".py", ".hip", ".hpp", ".h", ".cpp", ".jl")):
             txt = open(f).read()
             assert "oracle" not in txt.lower(), f
+
+
+def _julia_struct(src, name):
+    """Field (name, Julia type) list of `struct name ... end` in the shim."""
+    import re
+
+    m = re.search(r"struct %s\n(.*?)\nend" % name, src, re.S)
+    assert m, name
+    out = []
+    for line in m.group(1).splitlines():
+        line = line.split("#")[0].strip()
+        if line:
+            f, t = line.split("::")
+            out.append((f.strip(), t.strip()))
+    return out
+
+
+def test_julia_shim_matches_the_c_abi():
+    """The Julia ccall binding (not runnable here: no Julia in the image) lays
+    its structs out like include/ethersph.h -- same fields, order and widths
+    as the ctypes binding, whose layout test_struct_layout_matches_c checks
+    against the C compiler -- and calls only entry points the header declares."""
+    import re
+
+    src = open(os.path.join(ROOT, "ethersphcpu.jl_amd", "julia", "EtherSPHMI355X.jl")).read()
+    jl_size = {"Int32": 4, "Int64": 8, "Float64": 8, "Ptr{Float64}": 8, "NTuple{3,Float64}": 24,
+               "NTuple{32,Float64}": 256}
+    for jname, cls in (("EsphSceneDesc", _lib.SceneDesc), ("EsphFluidArrays", _lib.FluidArrays),
+                       ("EsphWallArrays", _lib.WallArrays)):
+        jf = _julia_struct(src, jname)
+        cf = cls._fields_
+        assert [f for f, _ in jf] == [f for f, _ in cf], jname
+        for (f, t), (_, ct) in zip(jf, cf):
+            assert jl_size[t] == ctypes.sizeof(ct), (jname, f, t)
+    called = set(re.findall(r"ccall\(\(:(esph_\w+), libesph\)", src))
+    assert called and called <= set(declared_symbols()), called - set(declared_symbols())

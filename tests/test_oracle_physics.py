@@ -265,7 +265,7 @@ def test_exact_cutoff_lattice_pairs():
 def test_check_planes_delete_like_deleteat():
     """A general check (box plus tilted half-spaces) on free particles:
     the survivors after N steps are those a replay of check + deleteat! on
-    Python lists keeps, in order (FluidWallSolve.jl:89-97)."""
+    Python lists keeps, in order (FluidWallSolve.jl:73-81)."""
     rng = np.random.default_rng(11)
     n = 60
     sc = lattice_scene(n=1, dx=0.001, check_interval=3,
