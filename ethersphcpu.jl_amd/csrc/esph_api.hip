@@ -70,6 +70,17 @@ bool kernel_params(int kind, int dim, double R, double* h, double* sigma) {
 }  // namespace esph
 
 // ---------------------------------------------------------------------------
+// Staging of esph_scene_probe_noalloc: the probe points (dim doubles each)
+// and the interpolated values (up to dim doubles each) of at most
+// kProbeNoallocPoints points, in a static device array (3 MiB), so that
+// entry point needs no allocation.  Each device holds its own copy at its own
+// address, resolved once per context at esph_ctx_create with the context's
+// device current; host code uses that address, never the array's name.
+// ---------------------------------------------------------------------------
+constexpr int64_t kProbeNoallocPoints = 65536;
+static __device__ __align__(256) double scratch[kProbeNoallocPoints * 6];
+
+// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 extern "C" int esph_abi_version(void) { return ESPH_ABI_VERSION; }
@@ -84,6 +95,11 @@ extern "C" int esph_ctx_create(int32_t device, uint64_t hbm_budget_bytes, esph_c
     esph_ctx* c = new esph_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
+    {
+        void* p = nullptr;
+        c->probe_scratch_err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        c->probe_scratch = c->probe_scratch_err == hipSuccess ? static_cast<double*>(p) : nullptr;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return ESPH_ERR_HIP;
@@ -1079,6 +1095,36 @@ extern "C" int esph_scene_probe(esph_scene* s, int32_t field, int64_t npoints, c
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     a.release();
     if (e != hipSuccess) return hip_fail(ctx, "probe", e);
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_probe_noalloc(esph_scene* s, int32_t field, int64_t npoints, const double* points,
+                                        double none_value, double* out) {
+    if (!s || npoints < 0 || (npoints > 0 && (!points || !out)))
+        return fail(s ? s->ctx : nullptr, ESPH_ERR_ARG, "probe_noalloc: bad argument");
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    const bool thermal = s->h.flags & FLAG_THERMAL;
+    if (field != ESPH_FIELD_V && field != ESPH_FIELD_RHO && field != ESPH_FIELD_P && field != ESPH_FIELD_C &&
+        !(field == ESPH_FIELD_T && thermal))
+        return fail(s->ctx, ESPH_ERR_ARG, "probe_noalloc: field must be V, RHO, P, C or (thermal solvers) T");
+    if (npoints > kProbeNoallocPoints)
+        return fail(s->ctx, ESPH_ERR_CAPACITY,
+                    "probe_noalloc: at most " + std::to_string(kProbeNoallocPoints) + " points per call");
+    esph_ctx* ctx = s->ctx;
+    if (!ctx->probe_scratch) return hip_fail(ctx, "probe_noalloc: staging address", ctx->probe_scratch_err);
+    if (npoints == 0) return ESPH_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int D = s->d.dim;
+    const int w = field == ESPH_FIELD_V ? D : 1;
+    hipStream_t st = ctx->stream;
+    double* dp = ctx->probe_scratch;                              // points: npoints * D
+    double* dout = ctx->probe_scratch + kProbeNoallocPoints * 3;  // values: npoints * w
+    hipError_t e = hipMemcpyAsync(dp, points, sizeof(double) * (size_t)npoints * D, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_probe(scene_bucket(s), npoints, dp, field, none_value, dout, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, sizeof(double) * (size_t)npoints * w, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, "probe_noalloc", e);
     return ESPH_OK;
 }
 

@@ -20,6 +20,10 @@ struct esph_ctx {
     uint64_t budget = 0;           // 0: unlimited
     uint64_t used = 0;
     std::string err;
+    // device address of the static probe staging array on this context's
+    // device (esph_scene_probe_noalloc); null if it could not be resolved
+    double* probe_scratch = nullptr;
+    hipError_t probe_scratch_err = hipSuccess;
     // gather depth and workgroup size of the pair passes: tuned per bucket
     // class (dim, kernel, precision, thermal mix) at its first step, or fixed
     int tune = 1;

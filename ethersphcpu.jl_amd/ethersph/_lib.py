@@ -116,6 +116,7 @@ def lib() -> ctypes.CDLL:
         "esph_scene_neighbours": ([vp, c_int32, POINTER(c_int32), c_int64, POINTER(c_int64)], c_int32),
         "esph_scene_walk_pairs": ([vp, c_int32, POINTER(c_int32), c_int64, POINTER(c_int64)], c_int32),
         "esph_scene_probe": ([vp, c_int32, c_int64, DP, c_double, DP], c_int32),
+        "esph_scene_probe_noalloc": ([vp, c_int32, c_int64, DP, c_double, DP], c_int32),
         "esph_scene_enable_timing": ([vp, c_int32], c_int32),
         "esph_scene_phase_times": ([vp, DP, POINTER(c_int64)], c_int32),
         "esph_scene_stats": ([vp, POINTER(c_int64)], c_int32),

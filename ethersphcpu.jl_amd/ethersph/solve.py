@@ -317,6 +317,17 @@ class Scene:
         out = out[: P.shape[0] * w]
         return out.reshape(-1, w) if w > 1 else out
 
+    def probe_noalloc(self, points, field: int, none_value: float = float("nan")) -> np.ndarray:
+        """probe() through esph_scene_probe_noalloc: no device allocation, at
+        most 65,536 points per call."""
+        P = _lib.f64(np.asarray(points, dtype=np.float64).reshape(-1, self.dim))
+        w = self.dim if field == _lib.FIELD_V else 1
+        out = np.empty(max(P.shape[0] * w, 1))
+        self.ctx.check(self.L.esph_scene_probe_noalloc(self.handle, field, P.shape[0], _lib.dptr(P), none_value,
+                                                       _lib.dptr(out)))
+        out = out[: P.shape[0] * w]
+        return out.reshape(-1, w) if w > 1 else out
+
     def enable_timing(self, on: bool = True) -> None:
         self.ctx.check(self.L.esph_scene_enable_timing(self.handle, int(on)))
 

@@ -330,6 +330,16 @@ int esph_scene_walk_pairs(esph_scene* scene, int32_t pairset, int32_t* out_pairs
    _T.  Synchronous. */
 int esph_scene_probe(esph_scene* scene, int32_t field, int64_t npoints, const double* points, double none_value,
                      double* out);
+/* esph_scene_probe without any device allocation.  No counterpart in the
+   reference: for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+   points and values are staged in a static device array of the library
+   (3 MiB per GPU, held until the process ends), so npoints is at most 65,536
+   (ESPH_ERR_CAPACITY beyond).  Results equal esph_scene_probe's.  Every call
+   on one GPU shares that array: calls that overlap on one GPU (two host
+   threads, or contexts on two streams) race.  Synchronous. */
+int esph_scene_probe_noalloc(esph_scene* scene, int32_t field, int64_t npoints, const double* points,
+                             double none_value, double* out);
 /* Average device time per step of each phase over the steps since the last
    call (milliseconds): [0] cell sort, [1] continuity+EOS, [2] forces+integrate,
    [3] Shepard (per step it ran), [4] 0 (the check runs inside the sort);
