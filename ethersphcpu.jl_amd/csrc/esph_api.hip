@@ -401,6 +401,24 @@ static int cell_of_host(const SceneDev& hd, int D, const double* x) {
     return c;
 }
 
+// Whether the force pass may walk two records per candidate and take p / rho^2
+// from the equation of state (FLAG_EOS_WALK): 3D FP32_BULK, no heat, no
+// buffer particles (whose p need not follow the EOS), gamma >= 1 and one gap
+// for every fluid particle.
+static bool eos_walk_ok(const esph_scene* s, const esph_fluid_arrays* f) {
+    const esph_scene_desc& d = s->d;
+    const bool thermal = d.solver == ESPH_SOLVER_FLUID_THERMOSTATIC_WALL ||
+                         d.solver == ESPH_SOLVER_FLUID_COMPULSIVE_THERMOSTATIC_WALL;
+    if (d.dim != 3 || d.precision != ESPH_FP32_BULK || thermal || d.solver == ESPH_SOLVER_FLUID_INLET_OUTLET_WALL ||
+        d.gamma < 1 || f->n <= 0)
+        return false;
+    const double g0 = f->gap[0];
+    if (!std::isfinite(g0)) return false;
+    for (int64_t i = 1; i < f->n; ++i)
+        if (f->gap[i] != g0) return false;
+    return true;
+}
+
 // Allocates every device buffer of the scene (capacity cap) and fills the
 // static parts of SceneDev.
 static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
@@ -412,7 +430,8 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     // the inlet/outlet solver grows by inlet copies: default headroom of 2x
     int64_t cap64 = d.fluid_capacity > 0 ? d.fluid_capacity : (io ? 2 * n_all + 1024 : n_all);
     if (cap64 < n_all) return fail(ctx, ESPH_ERR_ARG, "fluid_capacity smaller than the upload");
-    if (cap64 > INT32_MAX / 4) return fail(ctx, ESPH_ERR_ARG, "fluid capacity too large");
+    // record gathers take 32-bit byte offsets of 16-byte records (ldrec)
+    if (cap64 >= ((int64_t)1 << 28)) return fail(ctx, ESPH_ERR_ARG, "fluid capacity too large (at most 2^28 - 1)");
     const int cap = (int)std::max<int64_t>(cap64, 1);
     SceneDev hd{};
     hd.dim = D;
@@ -425,7 +444,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     // (FluidWallSolve.jl:68-79, FluidInletOutletWallSolve.jl:121-122)
     const bool wall_p = d.solver != ESPH_SOLVER_FLUID_WALL && !io;
     hd.flags = (thermal ? FLAG_THERMAL : 0) | (wall_p ? FLAG_WALL_PRESSURE : 0) | (d.check_enabled && !io ? FLAG_CHECK : 0) |
-               (io ? FLAG_IO : 0);
+               (io ? FLAG_IO : 0) | (eos_walk_ok(s, f) ? FLAG_EOS_WALK : 0);
     hd.io_axis = d.io_axis;
     hd.inlet_enter = d.inlet_enter;
     hd.inlet_shift = d.inlet_shift;
@@ -469,6 +488,10 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.c0 = d.c_0;
     hd.p0 = d.p_0;
     hd.b = d.c_0 * d.c_0 * d.rho_0 / (double)d.gamma;  // LiquidModel.jl:30
+    hd.rho0f = (float)d.rho_0;
+    hd.inv_rho0f = (float)(1.0 / d.rho_0);
+    hd.bf = (float)hd.b;
+    hd.p0f = (float)d.p_0;
     hd.nu0 = d.mu_0 / d.rho_0;                          // LiquidModel.jl:31
     hd.t0 = d.t_0;
     hd.cp = d.cp;
@@ -719,6 +742,14 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
         int rc = allocate_scene(s, f);
         if (rc) return rc;
         HIPCHK(ctx, hipMemcpy(s->dev, &s->h, sizeof(SceneDev), hipMemcpyHostToDevice));
+    } else if (((s->h.flags & FLAG_EOS_WALK) != 0) != eos_walk_ok(s, f)) {
+        // a re-upload that changes the gaps' uniformity changes the force walk
+        if (s->batch_refs > 0)
+            return fail(ctx, ESPH_ERR_STATE, "set_fluid: the new gaps change the scene's force walk while it is in a batch");
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        s->h.flags ^= FLAG_EOS_WALK;
+        HIPCHK(ctx, hipMemcpy(s->dev, &s->h, sizeof(SceneDev), hipMemcpyHostToDevice));
+        s->gplan.release();
     }
     SceneDev& hd = s->h;
     const bool io = (hd.flags & FLAG_IO) != 0;
@@ -786,7 +817,10 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
 
 static Bucket scene_bucket(esph_scene* s) {
     const bool th = (s->h.flags & FLAG_THERMAL) != 0;
-    return Bucket{s->d.dim, s->d.kernel, s->d.precision, s->dev, s->ptiles, s->nptiles, s->otiles, 1, th, !th};
+    const bool eos = (s->h.flags & FLAG_EOS_WALK) != 0;
+    Bucket b{s->d.dim, s->d.kernel, s->d.precision, s->dev, s->ptiles, s->nptiles, s->otiles, 1, th, !th && !eos};
+    b.has_eos = eos;
+    return b;
 }
 
 static hipEvent_t timing_event(esph_scene* s) {

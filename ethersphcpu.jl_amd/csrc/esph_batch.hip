@@ -24,7 +24,7 @@ using namespace esph;
 namespace esph {
 
 std::array<int, 4> depth_key(const Bucket& b) {
-    return {b.dim, b.kernel, b.prec, (b.has_thermal ? 2 : 0) | (b.has_plain ? 1 : 0)};
+    return {b.dim, b.kernel, b.prec, (b.has_thermal ? 2 : 0) | (b.has_plain || b.has_eos ? 1 : 0)};
 }
 
 void apply_depth(const esph_ctx* ctx, Bucket& b) {
@@ -111,7 +111,7 @@ int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
     // per-bucket tile tables, laid out after the global ones in one buffer
     std::vector<Tile> tiles(all_p);
     tiles.insert(tiles.end(), all_c.begin(), all_c.end());
-    struct Off { size_t p, np, o, no; bool th, pl; int dim, kernel, prec, ns; };
+    struct Off { size_t p, np, o, no; bool th, pl, eo; int dim, kernel, prec, ns; };
     std::vector<Off> offs;
     for (auto& g : groups) {
         Off o{};
@@ -123,8 +123,10 @@ int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
         for (int k : g.second) {
             for (int t = 0; t < members[k]->nptiles; ++t) tiles.push_back(Tile{k, t * kBlock});
             const bool th = (members[k]->h.flags & FLAG_THERMAL) != 0;
+            const bool eo = (members[k]->h.flags & FLAG_EOS_WALK) != 0;
             o.th = o.th || th;
-            o.pl = o.pl || !th;
+            o.pl = o.pl || (!th && !eo);
+            o.eo = o.eo || eo;
         }
         o.np = tiles.size() - o.p;
         o.o = tiles.size();
@@ -150,6 +152,7 @@ int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
     for (const Off& o : offs) {
         buckets.push_back(Bucket{o.dim, o.kernel, o.prec, dtable, dtiles + o.p, (int)o.np, dtiles + o.o, o.ns, o.th,
                                  o.pl});
+        buckets.back().has_eos = o.eo;
         apply_depth(ctx, buckets.back());
     }
     plan = SortPlan{dtable, n, dtiles, (int)all_p.size(), dtiles + all_p.size(), (int)all_c.size()};

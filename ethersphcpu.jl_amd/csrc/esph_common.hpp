@@ -29,6 +29,31 @@ __device__ __forceinline__ T ld(const void* p, int i) {
     return static_cast<const T*>(p)[i];
 }
 
+// Record j of an array of 16-byte (or 8-byte) records, gathered through a raw
+// buffer resource (gfx9 descriptor, word 3 0x00020000) on the wave-uniform
+// base of a scene's array: each lane supplies a 32-bit byte offset (one
+// shift) instead of a 64-bit address (three instructions), and the load
+// counts only in vmcnt.  Offsets cover 2^28 records (the scene capacity is
+// held below that).  Every lane of a wavefront works on one scene (a tile is
+// one scene's), so the base is the same in all lanes; it is moved to scalar
+// registers as such.
+template <typename V>
+__device__ __forceinline__ V ldrec(const void* base, int j) {
+    const uint64_t a = reinterpret_cast<uint64_t>(base);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    void* ub = reinterpret_cast<void*>(((uint64_t)hi << 32) | lo);
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(ub, (short)0, -1, 0x00020000);
+    if constexpr (sizeof(V) == 16) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((uint32_t)j << 4), 0, 0);
+        return __builtin_bit_cast(V, v);
+    } else {
+        static_assert(sizeof(V) == 8, "16- or 8-byte records");
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)((uint32_t)j << 3), 0, 0);
+        return __builtin_bit_cast(V, v);
+    }
+}
+
 template <int D>
 __device__ __forceinline__ void load_x(const double* __restrict__ x, int j, double* o) {
     if (D == 2) {
@@ -396,29 +421,49 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
 }
 
 // Quantized positions (FluidSet::xq, FP32_BULK).
+// (w: rho - rho0 in 3D, see FluidSet::xq)
 template <int D>
-__device__ __forceinline__ void store_xq(const SceneDev& s, const FluidSet& N, int i, const double* x) {
+__device__ __forceinline__ void store_xq(const SceneDev& s, const FluidSet& N, int i, const double* x, float w) {
     int32_t q[3];
     quantize(s.lo, s.inv_qu, D, x, q);
     if (D == 3)
-        static_cast<int4*>(N.xq)[i] = make_int4(q[0], q[1], q[2], 0);
+        static_cast<int4*>(N.xq)[i] = make_int4(q[0], q[1], q[2], __float_as_int(w));
     else
         static_cast<int2*>(N.xq)[i] = make_int2(q[0], q[1]);
 }
 
+// q[0..2]: quantized position; q[3]: the w bits in 3D (0 in 2D)
 template <int D>
 __device__ __forceinline__ void load_q(const void* xq, int j, int32_t* q) {
     if (D == 3) {
-        const int4 v = static_cast<const int4*>(xq)[j];
+        const int4 v = ldrec<int4>(xq, j);
         q[0] = v.x;
         q[1] = v.y;
         q[2] = v.z;
+        q[3] = v.w;
     } else {
-        const int2 v = static_cast<const int2*>(xq)[j];
+        const int2 v = ldrec<int2>(xq, j);
         q[0] = v.x;
         q[1] = v.y;
         q[2] = 0;
+        q[3] = 0;
     }
+}
+
+// p / rho^2 of a fluid particle from d = rho - rho0 in float, the equation of
+// state of updatePressure! (SelfAction/UpdatePressure.jl:8-16) written in d
+// so that the small difference rho / rho0 - 1 keeps its precision:
+// (rho / rho0)^gamma - 1 = e (1 + u + ... + u^(gamma-1)), e = d / rho0, u = 1 + e;
+// p = b e (...) + p0.  Also returns rho = rho0 + d.
+__device__ __forceinline__ float eos_pr2(const SceneDev& s, float d, float& rho) {
+    const float e = d * s.inv_rho0f;
+    const float u = 1.0f + e;
+    float g = 1.0f;
+    for (int k = 1; k < s.gamma; ++k) g = g * u + 1.0f;
+    const float p = s.bf * (e * g) + s.p0f;
+    rho = s.rho0f + d;
+    const float ir = prcp(rho);
+    return p * ir * ir;
 }
 
 // Pair geometry from quantized positions: rv = (q_i - q_j) qu (the integer
@@ -452,14 +497,15 @@ __device__ __forceinline__ bool qgeom(const SceneDev& s, const double* __restric
 
 // The list walk of for_pairs_d with the candidates' geometry from their
 // quantized positions (one 16-byte or 8-byte load instead of the binary64
-// position): body(j, in, d2, rv, record) with in = the FP64 rule's decision.
+// position): body(j, in, d2, rv, record, w) with in = the FP64 rule's
+// decision and w the candidate's FluidSet::xq w bits.
 // Gather groups as in for_pairs_d (SPLIT: the positions, then the records).
 template <int D, int DEPTH, bool SPLIT = true, typename Load, typename Body>
 __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S, const double* xi, int i,
                                             const int32_t* __restrict__ list, int cnt, Load&& load, Body&& body) {
     static_assert(DEPTH == 1 || DEPTH == 2 || DEPTH == 4, "DEPTH divides a 4-entry chunk");
     if (cnt <= 0) return;
-    int32_t qi[3];
+    int32_t qi[4];
     load_q<D>(S.xq, i, qi);
     const size_t cap = (size_t)s.cap;
     const int nq = (cnt + 3) >> 2;
@@ -478,7 +524,7 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
             if (q + 3 < nq) n1 = ld_chunk(list + list_at(cap, 4 * (q + 3), i));
             const int js[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
             const int m = cnt - 4 * q;  // entries of these two chunks: min(8, m)
-            int32_t qj[8][3];
+            int32_t qj[8][4];
             decltype(load(0)) rec[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) load_q<D>(S.xq, js[u], qj[u]);
@@ -489,7 +535,7 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
                 if (u < m) {
                     float rv[3], d2;
                     const bool in = qgeom<D>(s, S.x, xi, qi, qj[u], js[u], rv, d2);
-                    body(js[u], in, d2, rv, rec[u]);
+                    body(js[u], in, d2, rv, rec[u], qj[u][3]);
                 }
             }
             c0 = n0;
@@ -506,7 +552,7 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
 #pragma unroll
         for (int g = 0; g < 4; g += DEPTH) {
             if (g < m) {
-                int32_t qj[DEPTH][3];
+                int32_t qj[DEPTH][4];
                 decltype(load(0)) rec[DEPTH];
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) load_q<D>(S.xq, js[g + u], qj[u]);
@@ -521,7 +567,7 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
                     if (g + u < m) {
                         float rv[3], d2;
                         const bool in = qgeom<D>(s, S.x, xi, qi, qj[u], js[g + u], rv, d2);
-                        body(js[g + u], in, d2, rv, rec[u]);
+                        body(js[g + u], in, d2, rv, rec[u], qj[u][3]);
                     }
                 }
             }
@@ -567,7 +613,11 @@ struct Rec {
 template <int D, typename T>
 __device__ __forceinline__ void load_A(const void* A, int j, T* v, T& m, T& gap) {
     using V = typename Vec4<T>::type;
-    const V a = static_cast<const V*>(A)[j];
+    V a;
+    if constexpr (sizeof(V) == 16)
+        a = ldrec<V>(A, j);
+    else
+        a = static_cast<const V*>(A)[j];
     v[0] = a.x;
     v[1] = a.y;
     if (D == 3) {
@@ -617,7 +667,11 @@ struct RecLoaderAB {
             asm volatile("" : "+v"(j) : "v"(r.v[0]), "v"(r.v[1]), "v"(r.v[2]), "v"(r.m));
         else
             asm volatile("" : "+v"(j) : "v"(r.v[0]), "v"(r.v[1]), "v"(r.m), "v"(r.gap));
-        const V b = static_cast<const V*>(S.B)[j];
+        V b;
+        if constexpr (sizeof(V) == 16)
+            b = ldrec<V>(S.B, j);
+        else
+            b = static_cast<const V*>(S.B)[j];
         r.rho = b.x;
         r.pr2 = b.y;
         if (D == 3) {

@@ -20,6 +20,8 @@ constexpr int kBlock = 256;          // particles per tile == threads per block
 constexpr int kScanItems = 16;
 constexpr int kCellTile = kBlock * kScanItems;
 constexpr int kOvfTiles = 16;        // tiles per scene of the overflow (cell-scan) kernels
+// waves per SIMD the force pass of FLAG_EOS_WALK scenes is compiled for
+constexpr int kEosWalkWaves = 4;
 
 // SceneDev::flags
 enum : int {
@@ -27,6 +29,12 @@ enum : int {
     FLAG_WALL_PRESSURE = 2,  // pressureForce!(fluid, wall) in the wall loop
     FLAG_CHECK = 4,          // AABB deletion predicate enabled
     FLAG_IO = 8,             // inlet/outlet solver: buffer particles and migration
+    // 3D FP32_BULK scene without buffer particles, heat or varying gaps: the
+    // force pass's fluid walk gathers two records per candidate (quantized
+    // position with rho - rho0, velocity with mass) and takes p / rho^2 from
+    // the equation of state (every fluid particle's p is EOS(rho) once the
+    // continuity pass has run)
+    FLAG_EOS_WALK = 16,
 };
 
 // Particle kinds of the inlet/outlet solver (FluidSet::kind).
@@ -103,11 +111,13 @@ struct FluidSet {
     // only inside a rounding band around R
     void* xf;
     // FP32_BULK only (null otherwise): positions relative to the grid origin
-    // in whole quanta SceneDev::qu (int4 in 3D with w = 0, int2 in 2D; every
-    // component kQInvalid when a coordinate is beyond 2^30 quanta or not
+    // in whole quanta SceneDev::qu (int4 in 3D, int2 in 2D; the position
+    // components all kQInvalid when a coordinate is beyond 2^30 quanta or not
     // finite).  Differences of these are exact, so the continuity pass and
     // the 3D force pass's fluid walk take their pair geometry from one
-    // 16-byte load per candidate (see qgeom).
+    // 16-byte load per candidate (see qgeom).  In 3D, w holds rho - rho0 as
+    // float bits, rewritten in place by the continuity pass (the density the
+    // force pass uses; see FLAG_EOS_WALK).
     void* xq;
 };
 
@@ -150,6 +160,8 @@ struct SceneDev {
     // the quantized float distance cannot decide the FP64 rule (see qgeom)
     double qu, inv_qu;
     float quf, qbandf, R2f, pad5;
+    // the equation of state in float for FLAG_EOS_WALK (see eos_pr2)
+    float rho0f, inv_rho0f, bf, p0f;
     // fluid: set[2 * parity + dpar] (see FluidSet)
     FluidSet set[4];
     double* rhon;    // Shepard output / upload staging

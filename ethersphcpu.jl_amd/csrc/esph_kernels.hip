@@ -385,17 +385,18 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
                                                         (float)(x[2] - s.lo[2]), 0.0f);
         else
             static_cast<float2*>(O.xf)[p] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
-        if (O.xq) {
-            int32_t qx[3];
-            quantize(s.lo, s.inv_qu, D, x, qx);
-            if (D == 3)
-                static_cast<int4*>(O.xq)[p] = make_int4(qx[0], qx[1], qx[2], 0);
-            else
-                static_cast<int2*>(O.xq)[p] = make_int2(qx[0], qx[1]);
-        }
         for (int k = 0; k < D; ++k) s.xref[p * D + k] = x[k];  // the candidate lists are rebuilt from here
         const bool rho_new = mode == 1 || (shepard_due(s, dyn) && src_fluid && !spawn);
         const double rho = rho_new ? s.rhon[src] : S.rho[src];
+        if (O.xq) {
+            // 3D: w = rho - rho0 (FluidSet::xq)
+            int32_t qx[3];
+            quantize(s.lo, s.inv_qu, D, x, qx);
+            if (D == 3)
+                static_cast<int4*>(O.xq)[p] = make_int4(qx[0], qx[1], qx[2], __float_as_int((float)(rho - s.rho0)));
+            else
+                static_cast<int2*>(O.xq)[p] = make_int2(qx[0], qx[1]);
+        }
         const double pp = S.p[src];
         O.rho[p] = rho;
         O.p[p] = pp;
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_pairs(const SceneDev* __restric
                                       [&](int j, double, const double*) { emit(S.id[j]); });
             } else if constexpr (sizeof(T) == 4) {
                 for_pairs_q<D, 1>(s, S, xi, i, s.vnbr, s.nvnb[i], [](int) { return 0; },
-                                  [&](int j, bool in, float, const float*, int) {
+                                  [&](int j, bool in, float, const float*, int, int32_t) {
                                       if (in) emit(S.id[j]);
                                   });
             } else {

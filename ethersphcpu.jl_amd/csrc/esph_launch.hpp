@@ -19,7 +19,8 @@ struct Bucket {
     const Tile* ovf_tiles;   // device: kOvfTiles tiles per scene over SceneDev::ovf
     int nscenes;
     bool has_thermal;        // some scene of the bucket is thermal
-    bool has_plain;          // some scene of the bucket is not
+    bool has_plain;          // some scene of the bucket is neither thermal nor FLAG_EOS_WALK
+    bool has_eos = false;    // some scene of the bucket has FLAG_EOS_WALK
     int depth = 2;           // gather depth of the pair passes (1, 2 or 4; see the tuner)
     int bs = 256;            // workgroup size of the pair passes (64, 128 or 256; see the tuner)
 };

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
         if constexpr (sizeof(T) == 4 && !SCAN) {
             // FP32_BULK: geometry from the quantized positions (FP64 rule near R)
             for_pairs_q<D, DEPTH>(s, S, xi, i, s.vnbr, cnt, load,
-                                  [&](int, bool in, float d2, const float* rv, const ARec<T>& a) {
+                                  [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t) {
                 pair(in ? T(1) : T(0), d2, rv, a);
             });
         } else {
@@ -108,6 +108,8 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
             T* B = static_cast<T*>(S.B) + 4 * (size_t)i;
             B[0] = (T)rho;
             B[1] = (T)(p / (rho * rho));
+            // the density the force pass's two-record walk reads (FluidSet::xq)
+            if (D == 3 && S.xq) reinterpret_cast<float*>(S.xq)[4 * (size_t)i + 3] = (float)(rho - s.rho0);
         }
         });
     }
@@ -137,9 +139,11 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
 // The float positions (FluidSet::xf) are read only by the list rebuild,
 // which follows a sort, and the sort writes them for its new order; the
 // per-step copies therefore carry the quantized positions alone.
+// (rho: the density the next continuity pass starts from, for xq.w)
 template <int D>
-__device__ __forceinline__ void store_next_q(const SceneDev& s, const FluidSet& N, int i, const double* x) {
-    if (N.xq) store_xq<D>(s, N, i, x);
+__device__ __forceinline__ void store_next_q(const SceneDev& s, const FluidSet& N, int i, const double* x,
+                                             double rho) {
+    if (N.xq) store_xq<D>(s, N, i, x, (float)(rho - s.rho0));
 }
 
 // Buffer particles of the inlet/outlet solver: dv = 0, then updatePosition!
@@ -158,7 +162,7 @@ __device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S
         x[k] = S.x[i * D + k] + (v * dt - 0.0 * (dt * dt) / 2.0);
         N.x[i * D + k] = x[k];
     }
-    store_next_q<D>(s, N, i, x);
+    store_next_q<D>(s, N, i, x, S.rho[i]);
     static_cast<V*>(N.A)[i] = static_cast<const V*>(S.A)[i];
     static_cast<V*>(N.B)[i] = static_cast<const V*>(S.B)[i];
     if (N.T) N.T[i] = S.T[i];
@@ -170,7 +174,7 @@ __device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S
 // particle order (the first step after a sort), so B is written whole;
 // otherwise B.zw there still hold this order's gap (and, without heat, T) and
 // B.xy is rewritten by the next continuity pass before any read.
-template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH>
+template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH, bool EOSW>
 __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet& S, const FluidSet& N, int i,
                                               bool full_records) {
     using KQ = KernelQ<T, K>;
@@ -185,7 +189,8 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     load_x<D>(S.x, i, xi);
     Rec<T> me;
     load_rec<D, T, THERMAL>(S, i, me);
-    const T rho_i = me.rho, pr2_i = me.pr2, gap_i = me.gap, m_i = me.m, T_i = me.Tt, kap_i = me.kap;
+    const T rho_i = me.rho, gap_i = me.gap, m_i = me.m, T_i = me.Tt, kap_i = me.kap;
+    T pr2_i = me.pr2;  // the EOS form's own value in the two-record walk below
     T W_gap, dW_gap;
     KQ::wdw(gap_i * inv_h, kA, kB, W_gap, dW_gap);
     const T inv_Wgap_i = prcp(W_gap);
@@ -199,8 +204,9 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     // positions then the records; float, both together
     constexpr bool SPLIT = sizeof(T) == 8;
     // one candidate: in = 1 for a neighbour under the exact rule, else 0;
-    // zero: coincident positions (normalize(0), Neighbour.jl:54-59)
-    auto ff = [&](T in, bool zero, T d2, const T* rv, const Rec<T>& o) {
+    // zero: coincident positions (normalize(0), Neighbour.jl:54-59);
+    // uniform: the scene's gaps are all equal, so W((gap_i + gap_j) / 2) = W(gap_i)
+    auto ff = [&](T in, bool zero, T d2, const T* rv, const Rec<T>& o, auto uniform) {
         const T r = psqrt(d2);
         T W, dW;
         KQ::wdw(r * inv_h, kA, kB, W, dW);
@@ -208,7 +214,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         // pressure, with the 1% correction W(r) / W((gap_i + gap_j) / 2)
         T P = pr2_i + o.pr2;
         T inv_Wg = inv_Wgap_i;
-        if (o.gap != gap_i) {
+        if (!decltype(uniform)::value && o.gap != gap_i) {
             T w2, d2w;
             KQ::wdw((gap_i + o.gap) * T(0.5) * inv_h, kA, kB, w2, d2w);
             inv_Wg = prcp(w2);
@@ -223,20 +229,44 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         if (THERMAL)
             dT += T(4) * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e) * in;
     };
-    if constexpr (sizeof(T) == 4 && !SCAN && D == 3) {
+    if constexpr (sizeof(T) == 4 && !SCAN && D == 3 && !THERMAL && EOSW) {
+        {
+            // two records per candidate: the quantized position with
+            // rho - rho0 (FluidSet::xq), and the velocity with the mass (A);
+            // p / rho^2 from the equation of state, for the candidates and,
+            // alike, for this particle
+            float rho_e;
+            pr2_i = eos_pr2(s, reinterpret_cast<const float*>(S.xq)[4 * (size_t)i + 3], rho_e);
+            auto loadA = [&](int j) {
+                ARec<T> a;
+                load_A<D, T>(S.A, j, a.v, a.m, a.gap);
+                return a;
+            };
+            for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA,
+                                         [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t w) {
+                Rec<T> o;
+                o.v[0] = a.v[0];
+                o.v[1] = a.v[1];
+                o.v[2] = a.v[2];
+                o.m = a.m;
+                o.pr2 = eos_pr2(s, __int_as_float(w), o.rho);
+                ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::true_type{});
+            });
+        }
+    } else if constexpr (sizeof(T) == 4 && !SCAN && D == 3) {
         // FP32_BULK in 3D: geometry from the quantized positions (FP64 rule
         // near R).  The 2D ensemble measured 4% slower with them (1.90
         // against 1.83 ms per batched step) and keeps the binary64 ones.
         for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], RecLoaderAB<D, T, THERMAL>{S},
-                                     [&](int, bool in, float d2, const float* rv, const Rec<T>& o) {
-            ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o);
+                                     [&](int, bool in, float d2, const float* rv, const Rec<T>& o, int32_t) {
+            ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::false_type{});
         });
     } else {
         for_pairs_d<D, SCAN, DEPTH, false>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i],
                                            RecLoaderAB<D, T, THERMAL>{S},
                                            [&](int, double d2, const double* rv, const Rec<T>& o) {
             const T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
-            ff(d2 <= R2 ? T(1) : T(0), d2 == 0.0, (T)d2, rvt, o);
+            ff(d2 <= R2 ? T(1) : T(0), d2 == 0.0, (T)d2, rvt, o, std::false_type{});
         });
     }
     if (THERMAL) dT *= prcp(rho_i * (T)S.cp[i]);
@@ -358,7 +388,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 disp2 += e * e;
             }
         }
-        store_next_q<D>(s, N, i, xnew);
+        store_next_q<D>(s, N, i, xnew, (double)rho_i);
         // records of the new state; float records hold T - t0 (see k_sort_scatter)
         using V = typename Vec4<T>::type;
         const T Tr = THERMAL ? (T)(sizeof(T) == 4 ? Tn - s.t0 : Tn) : me.Tt;
@@ -386,9 +416,13 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
 
 // Float forces without heat are held to 4 waves per SIMD (128 VGPRs): the
 // gather pipeline otherwise takes a few registers more and falls to 3.
-template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH, int BS>
+// EOSW: the instance for scenes with FLAG_EOS_WALK (3D FP32_BULK without
+// heat), whose fluid walk gathers two records per candidate; the other
+// instances skip those scenes and it skips the rest.
+template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH, int BS, bool EOSW = false>
 __global__ __launch_bounds__(SCAN ? kBlock : BS)
-__attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value && !THERMAL && !SCAN ? 4 : 1))) void k_forces(const SceneDev* __restrict__ scenes,
+__attribute__((amdgpu_waves_per_eu(EOSW && !SCAN ? kEosWalkWaves
+                                   : std::is_same<T, float>::value && !THERMAL && !SCAN ? 4 : 1))) void k_forces(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, int ntiles) {
     constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
     const int nv = ntiles * SUB;
@@ -399,12 +433,13 @@ __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value && !THERMAL && 
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
         if (((s.flags & FLAG_THERMAL) != 0) != THERMAL) continue;
+        if (((s.flags & FLAG_EOS_WALK) != 0) != EOSW) continue;
         double dmax = 0.0;
         for_work<SCAN>(s, dyn, tile, first, [&](int i) {
             if (!SCAN && overflowed(s, i)) return;
             const double d2 =
-                forces_body<D, K, T, THERMAL, SCAN, DEPTH>(s, s.set[cur_set(dyn)], s.set[next_set(dyn)], i,
-                                                           dyn.rebuild != 0);
+                forces_body<D, K, T, THERMAL, SCAN, DEPTH, EOSW>(s, s.set[cur_set(dyn)], s.set[next_set(dyn)], i,
+                                                                 dyn.rebuild != 0);
             dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
         });
         if (s.verlet) {
@@ -521,14 +556,16 @@ hipError_t launch_continuity_d(const Bucket& b, hipStream_t st) {
     return e;
 }
 
-template <int D, bool TH>
+template <int D, bool TH, bool EW = false>
 hipError_t launch_forces_t(const Bucket& b, hipStream_t st) {
     hipError_t e = hipSuccess;
     ESPH_DISPATCH_K(b.kernel, {
         e = launch_both(b, st, [&](auto scan, const Tile* tl, int nt) {
             constexpr bool S = decltype(scan)::value;
             ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
-                if (b.prec == ESPH_FP64)
+                if constexpr (EW)
+                    ESPH_PAIR_LAUNCH(SINGLE_ARG(k_forces<D, K, float, TH, S, DP, BSZ, true>), b.scenes, tl, nt);
+                else if (b.prec == ESPH_FP64)
                     ESPH_PAIR_LAUNCH(SINGLE_ARG(k_forces<D, K, double, TH, S, DP, BSZ>), b.scenes, tl, nt);
                 else
                     ESPH_PAIR_LAUNCH(SINGLE_ARG(k_forces<D, K, float, TH, S, DP, BSZ>), b.scenes, tl, nt);
@@ -538,14 +575,16 @@ hipError_t launch_forces_t(const Bucket& b, hipStream_t st) {
     return e;
 }
 
+// One force-pass instance per kind of scene in the bucket: thermal, plain,
+// and (3D FP32_BULK) the two-record walk of FLAG_EOS_WALK scenes.
 template <int D>
 hipError_t launch_forces_d(const Bucket& b, hipStream_t st) {
-    if (b.has_thermal) {
-        hipError_t e = launch_forces_t<D, true>(b, st);
-        if (e != hipSuccess) return e;
-    }
-    if (b.has_plain) return launch_forces_t<D, false>(b, st);
-    return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (b.has_thermal) e = launch_forces_t<D, true>(b, st);
+    if (e == hipSuccess && b.has_plain) e = launch_forces_t<D, false>(b, st);
+    if constexpr (D == 3)
+        if (e == hipSuccess && b.has_eos && b.prec == ESPH_FP32_BULK) e = launch_forces_t<D, false, true>(b, st);
+    return e;
 }
 
 template <int D>
