@@ -269,7 +269,7 @@ extern "C" int esph_scene_set_walls(esph_scene* s, int32_t ngroups, const esph_w
             return fail(s->ctx, ESPH_ERR_ARG, "set_walls: thermostatic walls need t and kappa");
         nw += g[q].n;
     }
-    if (nw > INT32_MAX / 4) return fail(s->ctx, ESPH_ERR_ARG, "set_walls: too many wall particles");
+    if (nw >= ((int64_t)1 << 27)) return fail(s->ctx, ESPH_ERR_ARG, "set_walls: too many wall particles (at most 2^27 - 1)");
     WallHost& w = s->walls;
     w = WallHost();
     w.n = nw;
@@ -374,19 +374,18 @@ static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) 
     const int e = (int)std::ceil(std::log2(std::max(extent, 1e-300) / 1073741824.0));
     hd.qu = std::ldexp(1.0, e);
     hd.inv_qu = std::ldexp(1.0, -e);
-    hd.quf = (float)hd.qu;
     // band for the quantized float distance near R (d <= 1.01 R): each axis
     // of rv is off the binary64 difference by at most one quantum (two
     // half-quantum roundings) plus one float rounding of the integer
     // difference, dx = u + 2^-24 |rv|; then |d2 - d^2| <= 2 sqrt(D) d dx
-    // + D dx^2, the float sum adds D + 1 roundings of d^2 and R2f one of R^2.
+    // + D dx^2, the float sum adds D + 1 roundings of d^2 and R2q one of R^2.
     // Doubled for margin.
     const double eps = std::ldexp(1.0, -24), dd = 1.01 * hd.R;
     const double dx = hd.qu + eps * dd;
     const double band = 2.0 * std::sqrt((double)D) * dd * dx + D * dx * dx + (D + 1) * eps * dd * dd +
                         eps * hd.R2;
-    hd.qbandf = (float)(2.0 * band);
-    hd.R2f = (float)hd.R2;
+    hd.qbandq = (float)(2.0 * band * hd.inv_qu * hd.inv_qu);
+    hd.R2q = (float)(hd.R2 * hd.inv_qu * hd.inv_qu);
 }
 
 static int cell_of_host(const SceneDev& hd, int D, const double* x) {
@@ -430,8 +429,8 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     // the inlet/outlet solver grows by inlet copies: default headroom of 2x
     int64_t cap64 = d.fluid_capacity > 0 ? d.fluid_capacity : (io ? 2 * n_all + 1024 : n_all);
     if (cap64 < n_all) return fail(ctx, ESPH_ERR_ARG, "fluid_capacity smaller than the upload");
-    // record gathers take 32-bit byte offsets of 16-byte records (ldrec)
-    if (cap64 >= ((int64_t)1 << 28)) return fail(ctx, ESPH_ERR_ARG, "fluid capacity too large (at most 2^28 - 1)");
+    // gathers take 32-bit byte offsets (24 bytes per 3D position, urs)
+    if (cap64 >= ((int64_t)1 << 27)) return fail(ctx, ESPH_ERR_ARG, "fluid capacity too large (at most 2^27 - 1)");
     const int cap = (int)std::max<int64_t>(cap64, 1);
     SceneDev hd{};
     hd.dim = D;

@@ -29,21 +29,25 @@ __device__ __forceinline__ T ld(const void* p, int i) {
     return static_cast<const T*>(p)[i];
 }
 
-// Record j of an array of 16-byte (or 8-byte) records, gathered through a raw
-// buffer resource (gfx9 descriptor, word 3 0x00020000) on the wave-uniform
-// base of a scene's array: each lane supplies a 32-bit byte offset (one
-// shift) instead of a 64-bit address (three instructions), and the load
-// counts only in vmcnt.  Offsets cover 2^28 records (the scene capacity is
-// held below that).  Every lane of a wavefront works on one scene (a tile is
-// one scene's), so the base is the same in all lanes; it is moved to scalar
-// registers as such.
-template <typename V>
-__device__ __forceinline__ V ldrec(const void* base, int j) {
+// Gathers through a raw buffer resource (gfx9 descriptor, word 3 0x00020000)
+// on the wave-uniform base of a scene's array: each lane supplies a 32-bit
+// byte offset (one shift or multiply) instead of a 64-bit address (three
+// instructions), and the loads count only in vmcnt.  Every lane of a
+// wavefront works on one scene (a tile is one scene's), so the base is the
+// same in all lanes; it is moved to scalar registers as such.  Offsets stay
+// below 2^32: scenes hold fewer than 2^27 particles and walls.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t urs(const void* base) {
     const uint64_t a = reinterpret_cast<uint64_t>(base);
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-    void* ub = reinterpret_cast<void*>(((uint64_t)hi << 32) | lo);
-    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(ub, (short)0, -1, 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(((uint64_t)hi << 32) | lo), (short)0, -1,
+                                             0x00020000);
+}
+
+// Record j of an array of 16-byte (or 8-byte) records.
+template <typename V>
+__device__ __forceinline__ V ldrec(const void* base, int j) {
+    const __amdgpu_buffer_rsrc_t r = urs(base);
     if constexpr (sizeof(V) == 16) {
         const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((uint32_t)j << 4), 0, 0);
         return __builtin_bit_cast(V, v);
@@ -54,16 +58,21 @@ __device__ __forceinline__ V ldrec(const void* base, int j) {
     }
 }
 
+// Binary64 position j (D doubles per particle).
 template <int D>
 __device__ __forceinline__ void load_x(const double* __restrict__ x, int j, double* o) {
+    const __amdgpu_buffer_rsrc_t r = urs(x);
     if (D == 2) {
-        double2 q = reinterpret_cast<const double2*>(x)[j];
+        const double2 q = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)((uint32_t)j << 4), 0, 0));
         o[0] = q.x;
         o[1] = q.y;
     } else {
-        o[0] = x[3 * j];
-        o[1] = x[3 * j + 1];
-        o[2] = x[3 * j + 2];
+        const uint32_t off = (uint32_t)j * 24u;
+        const double2 q = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+        const double z = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(off + 16u), 0, 0));
+        o[0] = q.x;
+        o[1] = q.y;
+        o[2] = z;
     }
 }
 
@@ -466,34 +475,50 @@ __device__ __forceinline__ float eos_pr2(const SceneDev& s, float d, float& rho)
     return p * ir * ir;
 }
 
-// Pair geometry from quantized positions: rv = (q_i - q_j) qu (the integer
-// difference is exact; the float conversion and the power-of-two quantum
-// round at most once), d2 = |rv|^2 in float.  Against the FP64 rule
-// d^2 <= R^2 on the binary64 positions, |d2 - d^2| stays below qbandf
-// near R (two half-quanta per axis, the float roundings, R2f's own
-// rounding; derived where qbandf is set), so a candidate outside the band
-// is decided by the float distance and one inside it -- or with a position
-// beyond the quantized range -- by the FP64 rule itself.  Returns whether j
-// is a neighbour; rv and d2 are the float geometry of the pair.
+// Pair geometry from quantized positions, in units of the quantum qu:
+// rv = q_i - q_j (the integer difference is exact; its float conversion
+// rounds at most once), d2 = |rv|^2 in float.  In metres these are rv qu and
+// d2 qu^2 exactly (qu is a power of two), and the pair passes that take them
+// scale their length constants by powers of qu instead (QScale), so every
+// product rounds as it would in metres.  Against the FP64 rule d^2 <= R^2 on
+// the binary64 positions, |d2 - d^2 / qu^2| stays below qbandq near R (two
+// half-quanta per axis, the float roundings, R2q's own rounding; derived
+// where qbandq is set), so a candidate outside the band is decided by the
+// float distance and one inside it -- or with a position beyond the
+// quantized range -- by the FP64 rule itself.  Returns whether j is a
+// neighbour; rv and d2 are the float geometry of the pair in quanta.
 template <int D>
 __device__ __forceinline__ bool qgeom(const SceneDev& s, const double* __restrict__ xall, const double* xi,
                                       const int32_t* qi, const int32_t* qj, int j, float* rv, float& d2) {
 #pragma unroll
-    for (int k = 0; k < D; ++k) rv[k] = (float)(int32_t)((uint32_t)qi[k] - (uint32_t)qj[k]) * s.quf;
+    for (int k = 0; k < D; ++k) rv[k] = (float)(int32_t)((uint32_t)qi[k] - (uint32_t)qj[k]);
     if (D == 2) rv[2] = 0.0f;
     d2 = D == 3 ? rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2] : rv[0] * rv[0] + rv[1] * rv[1];
-    bool in = d2 <= s.R2f;
-    if (qi[0] == kQInvalid || qj[0] == kQInvalid || fabsf(d2 - s.R2f) <= s.qbandf) {
+    bool in = d2 <= s.R2q;
+    if (qi[0] == kQInvalid || qj[0] == kQInvalid || fabsf(d2 - s.R2q) <= s.qbandq) {
         double xj[3], rvd[3];
         load_x<D>(xall, j, xj);
         const double d2d = dist2_exact<D>(xi, xj, rvd);
         in = d2d <= s.R2;
 #pragma unroll
-        for (int k = 0; k < D; ++k) rv[k] = (float)rvd[k];
-        d2 = (float)d2d;
+        for (int k = 0; k < D; ++k) rv[k] = (float)(rvd[k] * s.inv_qu);
+        d2 = (float)(d2d * (s.inv_qu * s.inv_qu));
     }
     return in;
 }
+
+// Length constants of the pair arithmetic in the units of its geometry:
+// metres (binary64 positions) or quanta (qgeom).  With quanta, 1 / h is
+// multiplied by qu, 0.01 h^2 divided by qu^2, and factors of r / (r^2 + 0.01 h^2)
+// by 1 / qu: powers of two, so the products equal the metre ones.
+template <typename T>
+struct QScale {
+    T inv_h, eta2, len;  // 1 / h, 0.01 h^2, and the factor of one r / (r^2 + 0.01 h^2)
+    __device__ __forceinline__ QScale(const SceneDev& s, bool quanta)
+        : inv_h((T)(quanta ? s.inv_h * s.qu : s.inv_h)),
+          eta2((T)(quanta ? 0.01 * (s.h * s.h) * (s.inv_qu * s.inv_qu) : 0.01 * (s.h * s.h))),
+          len((T)(quanta ? s.inv_qu : 1.0)) {}
+};
 
 // The list walk of for_pairs_d with the candidates' geometry from their
 // quantized positions (one 16-byte or 8-byte load instead of the binary64

@@ -156,10 +156,11 @@ struct SceneDev {
     int32_t stencil;
     int32_t pad1;
     // position quantum of FluidSet::xq: a power of two with the grid extent
-    // below 2^30 quanta; qband: half-width of the band around R^2 inside which
-    // the quantized float distance cannot decide the FP64 rule (see qgeom)
+    // below 2^30 quanta; qbandq: half-width of the band around R2q = R^2 / qu^2
+    // inside which the quantized float distance cannot decide the FP64 rule
+    // (see qgeom), both in quanta^2
     double qu, inv_qu;
-    float quf, qbandf, R2f, pad5;
+    float qbandq, R2q, pad5, pad6;
     // the equation of state in float for FLAG_EOS_WALK (see eos_pr2)
     float rho0f, inv_rho0f, bf, p0f;
     // fluid: set[2 * parity + dpar] (see FluidSet)

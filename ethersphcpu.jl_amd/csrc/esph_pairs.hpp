@@ -68,13 +68,14 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
             load_A<D, T>(S.A, j, a.v, a.m, a.gap);
             return a;
         };
-        // one candidate: in = 1 for a neighbour under the exact rule, else 0
-        auto pair = [&](T in, T d2, const T* rv, const ARec<T>& a) {
+        // one candidate: in = 1 for a neighbour under the exact rule, else 0;
+        // ih = 1 / h in the units of d2 and rv (QScale)
+        auto pair = [&](T ih, T in, T d2, const T* rv, const ARec<T>& a) {
             const T* vj = a.v;
             const T m_j = a.m;
             const T r = psqrt(d2);
             T W, dW;
-            KQ::wdw(r * inv_h, kA, kB, W, dW);
+            KQ::wdw(r * ih, kA, kB, W, dW);
             // normalize(0) is NaN and the reference zeroes it (Neighbour.jl:54-59)
             const T sg = d2 == T(0) ? T(0) : dW * prcp(r);
             T dot = T(0);
@@ -84,15 +85,16 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
         };
         if constexpr (sizeof(T) == 4 && !SCAN) {
             // FP32_BULK: geometry from the quantized positions (FP64 rule near R)
+            const T ihq = QScale<T>(s, true).inv_h;
             for_pairs_q<D, DEPTH>(s, S, xi, i, s.vnbr, cnt, load,
                                   [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t) {
-                pair(in ? T(1) : T(0), d2, rv, a);
+                pair(ihq, in ? T(1) : T(0), d2, rv, a);
             });
         } else {
             for_pairs_d<D, SCAN, DEPTH, true, D == 3>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
                                         [&](int, double d2, const double* rv, const ARec<T>& a) {
                 T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
-                pair(d2 <= R2 ? T(1) : T(0), (T)d2, rvt, a);
+                pair(inv_h, d2 <= R2 ? T(1) : T(0), (T)d2, rvt, a);
             });
         }
         {
@@ -206,10 +208,11 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     // one candidate: in = 1 for a neighbour under the exact rule, else 0;
     // zero: coincident positions (normalize(0), Neighbour.jl:54-59);
     // uniform: the scene's gaps are all equal, so W((gap_i + gap_j) / 2) = W(gap_i)
-    auto ff = [&](T in, bool zero, T d2, const T* rv, const Rec<T>& o, auto uniform) {
+    // g: the length constants in the units of d2 and rv (QScale)
+    auto ff = [&](T in, bool zero, T d2, const T* rv, const Rec<T>& o, auto uniform, const QScale<T>& g) {
         const T r = psqrt(d2);
         T W, dW;
-        KQ::wdw(r * inv_h, kA, kB, W, dW);
+        KQ::wdw(r * g.inv_h, kA, kB, W, dW);
         const T sg = zero ? T(0) : dW * prcp(r);
         // pressure, with the 1% correction W(r) / W((gap_i + gap_j) / 2)
         T P = pr2_i + o.pr2;
@@ -221,14 +224,15 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         }
         P += fabs(P) * T(0.01) * W * inv_Wg;
         // viscosity
-        const T r2e = r * r + eta2;
-        const T vf = T(4) * nu0 * r * dW * prcp((rho_i + o.rho) * r2e) * in;
+        const T r2e = r * r + g.eta2;
+        const T vf = T(4) * nu0 * g.len * r * dW * prcp((rho_i + o.rho) * r2e) * in;
         const T ps = -P * sg * in;
 #pragma unroll
         for (int k = 0; k < D; ++k) dv[k] += o.m * (ps * rv[k] + vf * (me.v[k] - o.v[k]));
         if (THERMAL)
-            dT += T(4) * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e) * in;
+            dT += T(4) * g.len * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e) * in;
     };
+    const QScale<T> gq(s, true), gm(s, false);
     if constexpr (sizeof(T) == 4 && !SCAN && D == 3 && !THERMAL && EOSW) {
         {
             // two records per candidate: the quantized position with
@@ -250,7 +254,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 o.v[2] = a.v[2];
                 o.m = a.m;
                 o.pr2 = eos_pr2(s, __int_as_float(w), o.rho);
-                ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::true_type{});
+                ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::true_type{}, gq);
             });
         }
     } else if constexpr (sizeof(T) == 4 && !SCAN && D == 3) {
@@ -259,14 +263,14 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         // against 1.83 ms per batched step) and keeps the binary64 ones.
         for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], RecLoaderAB<D, T, THERMAL>{S},
                                      [&](int, bool in, float d2, const float* rv, const Rec<T>& o, int32_t) {
-            ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::false_type{});
+            ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::false_type{}, gq);
         });
     } else {
         for_pairs_d<D, SCAN, DEPTH, false>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i],
                                            RecLoaderAB<D, T, THERMAL>{S},
                                            [&](int, double d2, const double* rv, const Rec<T>& o) {
             const T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
-            ff(d2 <= R2 ? T(1) : T(0), d2 == 0.0, (T)d2, rvt, o, std::false_type{});
+            ff(d2 <= R2 ? T(1) : T(0), d2 == 0.0, (T)d2, rvt, o, std::false_type{}, gm);
         });
     }
     if (THERMAL) dT *= prcp(rho_i * (T)S.cp[i]);
@@ -284,8 +288,13 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         };
         auto wload = [&](int j) {
             WRec w;
-            w.wa = static_cast<const V*>(s.wA)[j];
-            w.wb = static_cast<const V*>(s.wB)[j];
+            if constexpr (sizeof(V) == 16) {
+                w.wa = ldrec<V>(s.wA, j);
+                w.wb = ldrec<V>(s.wB, j);
+            } else {
+                w.wa = static_cast<const V*>(s.wA)[j];
+                w.wb = static_cast<const V*>(s.wB)[j];
+            }
             w.kw = (D == 3 && THERMAL) ? ld<T>(s.wkap, j) : T(0);
             return w;
         };
