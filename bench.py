@@ -85,6 +85,12 @@ def time_gpu(sc, prec, steps, warmup, phase_steps=200):
     wg = dev.ctx.tuned_workgroup(sc.dim, sc.kernel.kind, prec, 1)
     if wg is not None:
         ph["workgroup"] = {"chosen": wg[0], "forces_ms_at_64_128_256_threads": wg[1]}
+    # each pass's own choice (continuity timed dry, Shepard whether due or not)
+    for q, name in ((1, "continuity"), (2, "shepard")):
+        t = dev.ctx.tuned_pass(sc.dim, sc.kernel.kind, prec, q, 1)
+        if t is not None:
+            ph[f"{name}_tuning"] = {"depth": t[0], "threads": t[1],
+                                    "ms_at_depth_1_2_4_x_threads_64_128_256": [round(v, 4) for v in t[2]]}
     n = dev.fluid_count()
     dev.close()
     return wall, ph, n

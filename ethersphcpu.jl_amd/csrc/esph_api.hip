@@ -128,6 +128,8 @@ extern "C" int esph_ctx_set_tuning(esph_ctx* ctx, int32_t mode, int32_t depth) {
     ctx->depth_ms.clear();
     ctx->block.clear();
     ctx->block_ms.clear();
+    ctx->pass_cfg.clear();
+    ctx->pass_ms.clear();
     return ESPH_OK;
 }
 
@@ -160,6 +162,37 @@ extern "C" int esph_ctx_tuned_depth(const esph_ctx* ctx, int32_t dim, int32_t ke
     if (ms3) {
         const auto& m = ctx->depth_ms.at(it->first);
         for (int q = 0; q < 3; ++q) ms3[q] = m[q];
+    }
+    return ESPH_OK;
+}
+
+extern "C" int esph_ctx_tuned_pass(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision,
+                                   int32_t thermal_mix, int32_t pass, int32_t* depth, int32_t* threads, float* ms9) {
+    if (!ctx || !depth || !threads || pass < 0 || pass > 2) return ESPH_ERR_ARG;
+    const std::array<int, 4> key{dim, kernel, precision, thermal_mix};
+    if (pass == 0) {
+        auto id = ctx->depth.find(key);
+        auto ib = ctx->block.find(key);
+        if (id == ctx->depth.end() || ib == ctx->block.end()) return ESPH_ERR_STATE;
+        *depth = id->second;
+        *threads = ib->second;
+        if (ms9) {
+            // depths 1, 2, 4 at 256 threads, then 64, 128, 256 threads at the chosen depth
+            const auto& a = ctx->depth_ms.at(key);
+            const auto& b = ctx->block_ms.at(key);
+            for (int q = 0; q < 3; ++q) ms9[q] = a[q];
+            for (int q = 0; q < 3; ++q) ms9[3 + q] = b[q];
+            for (int q = 6; q < 9; ++q) ms9[q] = 0.0f;
+        }
+        return ESPH_OK;
+    }
+    auto it = ctx->pass_cfg.find(key);
+    if (it == ctx->pass_cfg.end()) return ESPH_ERR_STATE;
+    *depth = it->second[pass - 1][0];
+    *threads = it->second[pass - 1][1];
+    if (ms9) {
+        const auto& m = ctx->pass_ms.at(key)[pass - 1];
+        for (int q = 0; q < 9; ++q) ms9[q] = m[q];
     }
     return ESPH_OK;
 }

@@ -29,14 +29,41 @@ std::array<int, 4> depth_key(const Bucket& b) {
 
 void apply_depth(const esph_ctx* ctx, Bucket& b) {
     if (!ctx->tune) {
-        b.depth = ctx->fixed_depth;
-        b.bs = ctx->fixed_block;
+        b.depth = b.cdepth = b.sdepth = ctx->fixed_depth;
+        b.bs = b.cbs = b.sbs = ctx->fixed_block;
         return;
     }
     auto it = ctx->depth.find(depth_key(b));
     b.depth = it == ctx->depth.end() ? 0 : it->second;
     auto ib = ctx->block.find(depth_key(b));
     b.bs = ib == ctx->block.end() ? kBlock : ib->second;
+    auto ip = ctx->pass_cfg.find(depth_key(b));
+    if (ip != ctx->pass_cfg.end()) {
+        b.cdepth = ip->second[0][0];
+        b.cbs = ip->second[0][1];
+        b.sdepth = ip->second[1][0];
+        b.sbs = ip->second[1][1];
+    } else {
+        b.cdepth = b.sdepth = b.depth > 0 ? b.depth : 2;
+        b.cbs = b.sbs = b.bs;
+    }
+}
+
+// Shortest of two timed launches (after one untimed) of a pass of b, in ms.
+template <typename F>
+static hipError_t time_pass(F&& launch, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float& t) {
+    hipError_t e = launch();
+    t = 1e30f;
+    for (int r = 0; r < 2 && e == hipSuccess; ++r) {
+        e = hipEventRecord(e0, st);
+        if (e == hipSuccess) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        t = std::min(t, ms);
+    }
+    return e;
 }
 
 // Shortest of three timed force passes of b (after one untimed), in ms.
@@ -88,11 +115,51 @@ hipError_t tune_depth(esph_ctx* ctx, Bucket& b, hipStream_t st) {
     for (int c = 0; c < 2; ++c)
         if (bbest[c] < bbest[kb]) kb = c;
     b.bs = bcand[kb];
+    // continuity (dry) and Shepard (forced) at every depth and workgroup size
+    std::array<std::array<int, 2>, 2> cfg{};
+    std::array<std::array<float, 9>, 2> pms{};
+    e0 = e1 = nullptr;
+    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    b.tuning = true;
+    for (int pass = 0; pass < 2 && e == hipSuccess; ++pass) {
+        int best = 0;
+        for (int c = 0; c < 9 && e == hipSuccess; ++c) {
+            int& d = pass == 0 ? b.cdepth : b.sdepth;
+            int& w = pass == 0 ? b.cbs : b.sbs;
+            d = dcand[c / 3];
+            w = bcand[c % 3];
+            e = time_pass([&] { return pass == 0 ? launch_continuity(b, st) : launch_shepard(b, st); }, st, e0, e1,
+                          pms[pass][c]);
+            if (pms[pass][c] < pms[pass][best]) best = c;
+        }
+        cfg[pass] = {dcand[best / 3], bcand[best % 3]};
+    }
+    b.tuning = false;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) {
+        b.cdepth = b.sdepth = b.depth;
+        b.cbs = b.sbs = b.bs;
+        return e;
+    }
+    b.cdepth = cfg[0][0];
+    b.cbs = cfg[0][1];
+    b.sdepth = cfg[1][0];
+    b.sbs = cfg[1][1];
     ctx->depth[depth_key(b)] = b.depth;
     ctx->depth_ms[depth_key(b)] = dbest;
     ctx->block[depth_key(b)] = b.bs;
     ctx->block_ms[depth_key(b)] = bbest;
+    ctx->pass_cfg[depth_key(b)] = cfg;
+    ctx->pass_ms[depth_key(b)] = pms;
     return hipSuccess;
+}
+
+std::vector<int> StepPlan::launch_config() const {
+    std::vector<int> v;
+    for (const Bucket& b : buckets) v.insert(v.end(), {b.depth, b.bs, b.cdepth, b.cbs, b.sdepth, b.sbs});
+    return v;
 }
 
 int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
@@ -222,11 +289,7 @@ int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
     }
     // a graph captured with other depths or workgroup sizes (tuning settings
     // changed) is recaptured
-    std::vector<int> depths;
-    for (const Bucket& b : buckets) {
-        depths.push_back(b.depth);
-        depths.push_back(b.bs);
-    }
+    const std::vector<int> depths = launch_config();
     if (depths != captured_depth) drop_graphs();
     for (int64_t t = t0; t < t0 + nsteps; ++t) {
         const int q = shepard_due(t) ? 1 : 0;

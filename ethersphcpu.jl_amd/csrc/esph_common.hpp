@@ -759,7 +759,7 @@ template <typename L>
 static hipError_t launch_both(const Bucket& b, hipStream_t st, L&& launch) {
     if (b.ntiles <= 0) return hipSuccess;
     launch(std::false_type{}, b.tiles, b.ntiles);
-    launch(std::true_type{}, b.ovf_tiles, kOvfTiles * b.nscenes);
+    if (!b.tuning) launch(std::true_type{}, b.ovf_tiles, kOvfTiles * b.nscenes);
     return hipGetLastError();
 }
 

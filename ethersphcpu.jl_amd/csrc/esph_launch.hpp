@@ -21,8 +21,14 @@ struct Bucket {
     bool has_thermal;        // some scene of the bucket is thermal
     bool has_plain;          // some scene of the bucket is neither thermal nor FLAG_EOS_WALK
     bool has_eos = false;    // some scene of the bucket has FLAG_EOS_WALK
-    int depth = 2;           // gather depth of the pair passes (1, 2 or 4; see the tuner)
-    int bs = 256;            // workgroup size of the pair passes (64, 128 or 256; see the tuner)
+    int depth = 2;           // gather depth of the force pass (1, 2 or 4; see the tuner)
+    int bs = 256;            // workgroup size of the force pass (64, 128 or 256; see the tuner)
+    int cdepth = 2, cbs = 256;  // the same for the continuity pass
+    int sdepth = 2, sbs = 256;  // and for the Shepard pass
+    // tuner runs: the main instances only, the continuity pass dry (its
+    // result to SceneDev::rhon, the state untouched), the Shepard pass
+    // whether or not a re-initialisation is due (its result is rhon anyway)
+    bool tuning = false;
 };
 
 // Tables for the dimension-agnostic sort kernels of a whole batch.

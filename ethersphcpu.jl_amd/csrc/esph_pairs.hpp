@@ -18,9 +18,11 @@ namespace esph {
 // sums equal those over the neighbours alone, and the wavefront stays
 // convergent.
 // ---------------------------------------------------------------------------
+// dry (tuner runs): the step's lists as they are, overflowed particles left
+// out, the new density to SceneDev::rhon only.
 template <int D, int K, typename T, bool SCAN, int DEPTH, int BS>
 __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDev* __restrict__ scenes,
-                                                       const Tile* __restrict__ tiles, int ntiles) {
+                                                       const Tile* __restrict__ tiles, int ntiles, int dry) {
     using KQ = KernelQ<T, K>;
     constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
     const int nv = ntiles * SUB;
@@ -35,6 +37,7 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
         if (S.kind && S.kind[i] != KIND_FLUID) {
             // inlet/outlet buffers keep rho and p (drho is reset,
             // FluidInletOutletWallSolve.jl:91-99) and need no lists
+            if (dry) return;
             s.nvnb[i] = 0;
             if (s.nw > 0) s.nnbw[i] = 0;
             return;
@@ -46,12 +49,13 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
         load_A<D, T>(S.A, i, vi, m_i, gap_i);
         int cnt = 0;
         if (!SCAN) {
-            if (dyn.rebuild || !s.verlet) {
+            if ((dyn.rebuild || !s.verlet) && !dry) {
                 // right after a sort: candidate lists within Rv from the cells
                 cnt = rebuild_lists<D>(s, S, xi, i);
             } else {
                 cnt = s.nvnb[i];
             }
+            if (dry && overflowed(s, i)) return;
             if (overflowed(s, i)) {
                 // finished by the SCAN instance of this kernel
                 s.ovf[atomicAdd(&s.dyn->novf, 1)] = i;
@@ -101,6 +105,10 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
 #pragma clang fp contract(off)
             double rho = S.rho[i];
             rho += (double)drho * s.dt;
+            if (dry) {
+                s.rhon[i] = rho;
+                return;
+            }
             const double rr = rho / s.rho0;
             const double p = s.b * (ipow(rr, s.gamma) - 1.0) + s.p0;
             const double cc = s.c0 * sqrt(ipow(rr, s.gamma - 1));
@@ -475,9 +483,10 @@ __attribute__((amdgpu_waves_per_eu(EOSW && !SCAN ? kEosWalkWaves
 // ReconstructValue.jl:22-57): Shepard filter over the step-start pairs with
 // the post-continuity density, in binary64 in every form.
 // ---------------------------------------------------------------------------
+// force (tuner runs): whether or not a re-initialisation is due.
 template <int D, int K, bool SCAN, int DEPTH, int BS>
 __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* __restrict__ scenes,
-                                                    const Tile* __restrict__ tiles, int ntiles) {
+                                                    const Tile* __restrict__ tiles, int ntiles, int force) {
     using KQ = KernelQ<double, K>;
     constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
     const int nv = ntiles * SUB;
@@ -487,7 +496,7 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* 
         const int first = tile.begin + (v % SUB) * BS;
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
-        if (s.kreinit <= 0 || (dyn.step + 1) % s.kreinit != 0) continue;
+        if (!force && (s.kreinit <= 0 || (dyn.step + 1) % s.kreinit != 0)) continue;
         for_work<SCAN>(s, dyn, tile, first, [&](int i) {
         if (!SCAN && overflowed(s, i)) return;
         const FluidSet& S = s.set[cur_set(dyn)];
@@ -532,16 +541,16 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* 
 }
 
 
-// The pair passes run in workgroups of b.bs threads (the tuner's choice of
-// 64, 128 or 256), kBlock / b.bs of them per particle tile; the overflow
-// (SCAN) instances always run kBlock-thread workgroups.
+// The pair passes run in workgroups of bs_ threads (the tuner's choice of
+// 64, 128 or 256 for the pass), kBlock / bs_ of them per particle tile; the
+// overflow (SCAN) instances always run kBlock-thread workgroups.
 #define ESPH_PAIR_LAUNCH(KERNEL_T, ...)                                                          \
     do {                                                                                        \
         if constexpr (S) {                                                                      \
             constexpr int BSZ = kBlock;                                                         \
             hipLaunchKernelGGL((KERNEL_T), dim3(nt), dim3(BSZ), 0, st, __VA_ARGS__);            \
         } else {                                                                                \
-            ESPH_DISPATCH_BS(b.bs, {                                                            \
+            ESPH_DISPATCH_BS(bs_, {                                                             \
                 hipLaunchKernelGGL((KERNEL_T), dim3(nt * (kBlock / BSZ)), dim3(BSZ), 0, st,      \
                                    __VA_ARGS__);                                                \
             });                                                                                 \
@@ -551,14 +560,15 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* 
 template <int D>
 hipError_t launch_continuity_d(const Bucket& b, hipStream_t st) {
     hipError_t e = hipSuccess;
+    const int bs_ = b.cbs, dry = b.tuning ? 1 : 0;
     ESPH_DISPATCH_K(b.kernel, {
         e = launch_both(b, st, [&](auto scan, const Tile* tl, int nt) {
             constexpr bool S = decltype(scan)::value;
-            ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
+            ESPH_DISPATCH_DEPTH(S ? 1 : b.cdepth, {
                 if (b.prec == ESPH_FP64)
-                    ESPH_PAIR_LAUNCH(SINGLE_ARG(k_continuity<D, K, double, S, DP, BSZ>), b.scenes, tl, nt);
+                    ESPH_PAIR_LAUNCH(SINGLE_ARG(k_continuity<D, K, double, S, DP, BSZ>), b.scenes, tl, nt, dry);
                 else
-                    ESPH_PAIR_LAUNCH(SINGLE_ARG(k_continuity<D, K, float, S, DP, BSZ>), b.scenes, tl, nt);
+                    ESPH_PAIR_LAUNCH(SINGLE_ARG(k_continuity<D, K, float, S, DP, BSZ>), b.scenes, tl, nt, dry);
             });
         });
     });
@@ -568,6 +578,7 @@ hipError_t launch_continuity_d(const Bucket& b, hipStream_t st) {
 template <int D, bool TH, bool EW = false>
 hipError_t launch_forces_t(const Bucket& b, hipStream_t st) {
     hipError_t e = hipSuccess;
+    const int bs_ = b.bs;
     ESPH_DISPATCH_K(b.kernel, {
         e = launch_both(b, st, [&](auto scan, const Tile* tl, int nt) {
             constexpr bool S = decltype(scan)::value;
@@ -599,11 +610,12 @@ hipError_t launch_forces_d(const Bucket& b, hipStream_t st) {
 template <int D>
 hipError_t launch_shepard_d(const Bucket& b, hipStream_t st) {
     hipError_t e = hipSuccess;
+    const int bs_ = b.sbs, force = b.tuning ? 1 : 0;
     ESPH_DISPATCH_K(b.kernel, {
         e = launch_both(b, st, [&](auto scan, const Tile* tl, int nt) {
             constexpr bool S = decltype(scan)::value;
-            ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
-                ESPH_PAIR_LAUNCH(SINGLE_ARG(k_shepard<D, K, S, DP, BSZ>), b.scenes, tl, nt);
+            ESPH_DISPATCH_DEPTH(S ? 1 : b.sdepth, {
+                ESPH_PAIR_LAUNCH(SINGLE_ARG(k_shepard<D, K, S, DP, BSZ>), b.scenes, tl, nt, force);
             });
         });
     });

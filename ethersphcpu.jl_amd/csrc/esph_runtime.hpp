@@ -33,6 +33,10 @@ struct esph_ctx {
     std::map<std::array<int, 4>, std::array<float, 3>> depth_ms;  // forces pass at depth 1, 2, 4 (256 threads)
     std::map<std::array<int, 4>, int> block;
     std::map<std::array<int, 4>, std::array<float, 3>> block_ms;  // forces pass at 64, 128, 256 threads
+    // continuity [0] and Shepard [1] passes: (depth, workgroup size) chosen
+    // and the times (ms) at depth {1, 2, 4} x {64, 128, 256} threads
+    std::map<std::array<int, 4>, std::array<std::array<int, 2>, 2>> pass_cfg;
+    std::map<std::array<int, 4>, std::array<std::array<float, 9>, 2>> pass_ms;
 };
 
 namespace esph {
@@ -80,6 +84,7 @@ struct StepPlan {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     std::vector<int> captured_depth;  // bucket depths and workgroup sizes the graphs were captured with
+    std::vector<int> launch_config() const;  // those of the buckets now
     int build(esph_ctx* ctx, const std::vector<esph_scene*>& members);
     hipError_t launch_step(hipStream_t st, bool shepard) const;
     bool shepard_due(int64_t t) const;  // some member's step t from now re-initialises its density
@@ -88,14 +93,17 @@ struct StepPlan {
     void release();
 };
 
-// Pair-pass tuner.  apply_depth sets b.depth and b.bs from the context
-// (fixed values, or the tuned ones of b's class; depth 0 when that class is
-// not tuned yet).  tune_depth times the force pass of b at gather depths 1,
-// 2 and 4 with 256-thread workgroups, then at the fastest depth with 64- and
-// 128-thread workgroups, on the current step's lists (the pass only writes
-// the step outputs, so repeating it changes nothing), and records the
-// fastest pair; it must follow the step's continuity pass on the stream.
-// Results depend on neither choice.
+// Pair-pass tuner.  apply_depth sets each pass's gather depth and workgroup
+// size in b from the context (fixed values, or the tuned ones of b's class;
+// b.depth 0 when that class is not tuned yet).  tune_depth times the force
+// pass of b at gather depths 1, 2 and 4 with 256-thread workgroups, then at
+// the fastest depth with 64- and 128-thread workgroups, on the current step's
+// lists (the pass only writes the step outputs, so repeating it changes
+// nothing); then the continuity pass (dry: its result to rhon, the state
+// untouched) and the Shepard pass (whether or not due; its output is rhon)
+// at every depth and workgroup size; and records the fastest of each.  It
+// must follow the step's continuity pass on the stream.  Results depend on
+// none of the choices.
 std::array<int, 4> depth_key(const Bucket& b);
 void apply_depth(const esph_ctx* ctx, Bucket& b);
 hipError_t tune_depth(esph_ctx* ctx, Bucket& b, hipStream_t st);

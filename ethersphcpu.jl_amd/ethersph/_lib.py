@@ -92,6 +92,8 @@ def lib() -> ctypes.CDLL:
         "esph_ctx_destroy": ([vp], c_int32),
         "esph_last_error": ([vp], c_char_p),
         "esph_ctx_memory_used": ([vp, POINTER(c_uint64)], c_int32),
+        "esph_ctx_tuned_pass": ([vp, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_int32),
+                                 POINTER(c_float)], c_int32),
         "esph_ctx_set_tuning": ([vp, c_int32, c_int32], c_int32),
         "esph_ctx_set_workgroup": ([vp, c_int32], c_int32),
         "esph_ctx_tuned_workgroup": ([vp, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_float)],
@@ -204,6 +206,17 @@ class Context:
         if rc != ESPH_OK:
             return None
         return int(d.value), [float(v) for v in ms]
+
+    def tuned_pass(self, dim: int, kernel: int, precision: int, pass_: int, thermal_mix: int = 1):
+        """(depth, threads, ms9) chosen for a pass (0 forces, 1 continuity, 2
+        Shepard) of a class, or None before it is tuned."""
+        d, t, ms = c_int32(), c_int32(), (c_float * 9)()
+        rc = self._L.esph_ctx_tuned_pass(self.handle, dim, kernel, precision, thermal_mix, pass_, ctypes.byref(d),
+                                         ctypes.byref(t), ms)
+        if rc == 5:
+            return None
+        self.check(rc)
+        return d.value, t.value, list(ms)
 
     def memory_used(self) -> int:
         v = c_uint64()

@@ -244,10 +244,11 @@ const char* esph_last_error(const esph_ctx* ctx);
    identical for every choice.  mode 1 (default): the first step of each class
    of scenes (dim, kernel, precision, thermal mix) times the force pass at the
    three depths (256 threads), then at the fastest depth with 64 and 128
-   threads, on that step's neighbour lists, and keeps the fastest; mode 0: the
-   given depth (and the workgroup size of esph_ctx_set_workgroup, 256 by
-   default) everywhere.  Clears earlier choices; applies to steps launched
-   afterwards. */
+   threads, on that step's neighbour lists, and keeps the fastest; then the
+   continuity and Shepard passes at all nine pairs, each keeping its own
+   (esph_ctx_tuned_pass); mode 0: the given depth (and the workgroup size of
+   esph_ctx_set_workgroup, 256 by default) everywhere.  Clears earlier
+   choices; applies to steps launched afterwards. */
 int esph_ctx_set_tuning(esph_ctx* ctx, int32_t mode, int32_t depth);
 /* The workgroup size (64, 128 or 256 threads) used in tuning mode 0. */
 int esph_ctx_set_workgroup(esph_ctx* ctx, int32_t threads);
@@ -261,6 +262,14 @@ int esph_ctx_tuned_depth(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32
    ESPH_ERR_STATE if not tuned yet. */
 int esph_ctx_tuned_workgroup(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision,
                              int32_t thermal_mix, int32_t* threads, float* ms3);
+/* The gather depth and workgroup size chosen for one pass of a class (pass
+   0: forces, 1: continuity, 2: Shepard) and, if ms9 is not NULL, the times
+   (ms) it was chosen from: for forces, depths 1, 2, 4 at 256 threads then
+   64, 128, 256 threads at the chosen depth (ms9[6..8] zero); for continuity
+   (timed dry, the state untouched) and Shepard, depth {1, 2, 4} x {64, 128,
+   256} threads, depth-major.  ESPH_ERR_STATE if not tuned yet. */
+int esph_ctx_tuned_pass(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision, int32_t thermal_mix,
+                        int32_t pass, int32_t* depth, int32_t* threads, float* ms9);
 /* Bytes of device memory currently held by scenes of this context. */
 int esph_ctx_memory_used(const esph_ctx* ctx, uint64_t* bytes);
 

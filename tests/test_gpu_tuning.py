@@ -30,19 +30,33 @@ def run(sc, prec, steps, tune, depth, graph=False, threads=256):
     s.step(steps)
     s.sync()
     out = {f: s.get(f) for f in FIELDS + ((_lib.FIELD_T,) if s.thermal else ())}
-    tuned = ctx.tuned_depth(sc.dim, sc.kernel.kind, prec, 2 if s.thermal else 1)
+    mix = 2 if s.thermal else 1
+    tuned = ctx.tuned_depth(sc.dim, sc.kernel.kind, prec, mix)
+    if tuned is not None:
+        tuned = (tuned[0], tuned[1], [ctx.tuned_pass(sc.dim, sc.kernel.kind, prec, q, mix) for q in (0, 1, 2)])
     s.close()
     ctx.close()
     return out, tuned
 
 
+def dam_break_every_step_shepard():
+    """Re-initialisation at every step, so the tuning step (the first) is one:
+    the tuner's dry continuity and forced Shepard runs write SceneDev::rhon
+    before the step's own Shepard pass and must leave no trace."""
+    sc = scenarios.dam_break_3d(dx=0.06)
+    sc.td.density_reinitialized_interval_ = 1
+    return sc
+
+
 @pytest.mark.parametrize("name,prec", [("dam_break_3d", _lib.FP32_BULK), ("thermal_cavity_2d", _lib.FP64),
-                                       ("poiseuille_2d", _lib.FP32_BULK), ("lid_driven_cavity_2d", _lib.FP64)])
+                                       ("poiseuille_2d", _lib.FP32_BULK), ("lid_driven_cavity_2d", _lib.FP64),
+                                       ("dam_break_3d_shepard_each_step", _lib.FP32_BULK)])
 def test_results_do_not_depend_on_depth(name, prec):
     sc = {"dam_break_3d": lambda: scenarios.dam_break_3d(dx=0.06),
           "thermal_cavity_2d": lambda: scenarios.thermal_cavity_2d(dx=0.03, seed=3),
           "poiseuille_2d": lambda: scenarios.poiseuille_2d(rows=10, length_factor=2),
-          "lid_driven_cavity_2d": lambda: scenarios.lid_driven_cavity_2d(dx=0.04)}[name]()
+          "lid_driven_cavity_2d": lambda: scenarios.lid_driven_cavity_2d(dx=0.04),
+          "dam_break_3d_shepard_each_step": dam_break_every_step_shepard}[name]()
     say("depth independence", name, prec)
     ref, _ = run(sc, prec, 40, False, 1)
     for depth in (2, 4):
@@ -55,6 +69,10 @@ def test_results_do_not_depend_on_depth(name, prec):
             assert np.array_equal(got[f], ref[f]), (threads, f)
     got, tuned = run(sc, prec, 40, True, 2)
     assert tuned is not None and tuned[0] in (1, 2, 4) and all(t > 0 for t in tuned[1])
+    for q in (0, 1, 2):  # each pass's own choice, from its own timings
+        d, t, ms = tuned[2][q]
+        assert d in (1, 2, 4) and t in (64, 128, 256)
+        assert all(v > 0 for v in (ms[:6] if q == 0 else ms)), (q, ms)
     say("tuned", tuned)
     for f in ref:
         assert np.array_equal(got[f], ref[f]), ("tuned", f)
