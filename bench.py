@@ -127,15 +127,20 @@ def ensemble_leg(steps, alone_steps, n_scenes, full_steps=5000):
             s.step(alone_steps)
             s.sync()
         t_plain = (time.perf_counter() - t0) / alone_steps
+        # with the scene's own captured step graphs, one scene at a time: each
+        # scene captures, warms up, runs its timed steps and releases its
+        # graphs before the next (at most two live graph executables)
+        t_alone = 0.0
         for s in scenes:
             s.enable_graph(True)
             s.step(2)
-        scenes[0].sync()
-        t0 = time.perf_counter()
-        for s in scenes:
+            s.sync()
+            t0 = time.perf_counter()
             s.step(alone_steps)
             s.sync()
-        t_alone = (time.perf_counter() - t0) / alone_steps
+            t_alone += time.perf_counter() - t0
+            s.enable_graph(False)
+        t_alone /= alone_steps
     for s in scenes:
         s.close()
     batched = n_scenes / (t_batch * full_steps)

@@ -555,6 +555,10 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
             for (int u = 0; u < 8; ++u) load_q<D>(S.xq, js[u], qj[u]);
 #pragma unroll
             for (int u = 0; u < 8; ++u) rec[u] = load(after<D>(js[u], qj[u]));
+            if constexpr (has_late<std::decay_t<Load>>::value) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) load.late(js[u], rec[u]);
+            }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 if (u < m) {

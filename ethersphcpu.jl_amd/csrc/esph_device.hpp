@@ -67,6 +67,8 @@ struct SceneDyn {
     int32_t nsorts;       // sorts (list rebuilds) so far, the upload's included
     int32_t scan_done;    // cell-count tiles scanned in this sort (reset by the last one)
     int32_t outside_acc;  // particles far outside the grid, counted by the running sort
+    int32_t shepard_ran;  // 1: this step's Shepard pass wrote rhon (the sort moves it into rho)
+    int32_t pad_dyn;
 };
 
 // Index into SceneDev::set of the current state and of the step's outputs

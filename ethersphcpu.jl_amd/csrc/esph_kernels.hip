@@ -316,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
             // already in the other per-step copy (k_sort_finish switches to
             // it); only a Shepard step's densities move, into the order's rho
             const int i = tile.begin + (int)threadIdx.x;
-            if (shepard_due(s, dyn) && i < dyn.n_live) s.set[cur_set(dyn)].rho[i] = s.rhon[i];
+            if (dyn.shepard_ran && i < dyn.n_live) s.set[cur_set(dyn)].rho[i] = s.rhon[i];
             continue;
         }
         const bool io = mode == 0 && (s.flags & FLAG_IO);
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
         else
             static_cast<float2*>(O.xf)[p] = make_float2((float)(x[0] - s.lo[0]), (float)(x[1] - s.lo[1]));
         for (int k = 0; k < D; ++k) s.xref[p * D + k] = x[k];  // the candidate lists are rebuilt from here
-        const bool rho_new = mode == 1 || (shepard_due(s, dyn) && src_fluid && !spawn);
+        const bool rho_new = mode == 1 || (dyn.shepard_ran && src_fluid && !spawn);
         const double rho = rho_new ? s.rhon[src] : S.rho[src];
         if (O.xq) {
             // 3D: w = rho - rho0 (FluidSet::xq)
@@ -457,6 +457,7 @@ __global__ void k_sort_finish(const SceneDev* __restrict__ scenes, int nscenes, 
         d->rebuild = 0;
     }
     d->novf = 0;
+    d->shepard_ran = 0;
     d->nspawn = 0;
     d->nleave = 0;
     d->disp2_bits = 0;

@@ -497,6 +497,9 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* 
         const SceneDev& s = scenes[tile.scene];
         const SceneDyn dyn = *s.dyn;
         if (!force && (s.kreinit <= 0 || (dyn.step + 1) % s.kreinit != 0)) continue;
+        // the sort moves rhon into rho only when a scheduled pass ran (a
+        // tuner run or a skipped pass leaves the densities as they are)
+        if (!force && threadIdx.x == 0) s.dyn->shepard_ran = 1;
         for_work<SCAN>(s, dyn, tile, first, [&](int i) {
         if (!SCAN && overflowed(s, i)) return;
         const FluidSet& S = s.set[cur_set(dyn)];
