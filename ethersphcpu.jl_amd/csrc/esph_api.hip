@@ -140,6 +140,12 @@ extern "C" int esph_ctx_set_workgroup(esph_ctx* ctx, int32_t threads) {
     return ESPH_OK;
 }
 
+extern "C" int esph_ctx_set_brick_walk(esph_ctx* ctx, int32_t enable) {
+    if (!ctx) return ESPH_ERR_ARG;
+    ctx->bricks = enable != 0;
+    return ESPH_OK;
+}
+
 extern "C" int esph_ctx_tuned_workgroup(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision,
                                         int32_t thermal_mix, int32_t* threads, float* ms3) {
     if (!ctx || !threads) return ESPH_ERR_ARG;
@@ -451,6 +457,37 @@ static bool eos_walk_ok(const esph_scene* s, const esph_fluid_arrays* f) {
     return true;
 }
 
+// Brick shape of a FLAG_BRICK scene (cells per brick along x, y, z): the
+// largest brick whose region (widened by the stencil on each side) holds at
+// most 92% of kBrickCap particles and whose own particles fit one workgroup,
+// counted on a lattice of the given spacing (a window of w cells holds at
+// most floor(w cs / spacing) + 1 lattice points along an axis); among equal
+// volumes the longest along x (longer contiguous rows).  False if no brick
+// fits.
+static bool choose_bricks(const SceneDev& hd, double spacing, int bd[3]) {
+    if (!(spacing > 0.0) || !(hd.cs > 0.0)) return false;
+    const int m = hd.stencil;
+    const double a = hd.cs / spacing;
+    auto most = [&](int w) { return std::floor((double)w * a) + 1.0; };
+    int best = 0;
+    for (int bx = 1; bx <= 16; ++bx)
+        for (int by = 1; by <= 16; ++by)
+            for (int bz = 1; bz <= 16; ++bz) {
+                if ((by + 2 * m) * (bz + 2 * m) > kBrickRows || by * bz > kBrickOwn) continue;
+                const double region = most(bx + 2 * m) * most(by + 2 * m) * most(bz + 2 * m);
+                const double own = most(bx) * most(by) * most(bz);
+                if (region > 0.92 * kBrickCap || own > kBrickThreads) continue;
+                const int vol = bx * by * bz;
+                if (vol > best || (vol == best && bx > bd[0])) {
+                    best = vol;
+                    bd[0] = bx;
+                    bd[1] = by;
+                    bd[2] = bz;
+                }
+            }
+    return best > 0;
+}
+
 // Allocates every device buffer of the scene (capacity cap) and fills the
 // static parts of SceneDev.
 static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
@@ -554,6 +591,30 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         hd.kfw = nw > 0 ? std::min(std::max((k / 2 + 15) / 16 * 16, 16), 1024) : 0;
     }
 
+    // brick walks (FLAG_BRICK; taken up when FLAG_EOS_WALK holds): 3D
+    // FP32_BULK scenes without heat or buffers, bricks sized for the fluid's
+    // spacing (the first particle's gap)
+    size_t brick_bytes = 0;
+    {
+        int bd[3] = {0, 0, 0};
+        const double spacing = (f->n > 0 && f->gap) ? f->gap[0] : 0.0;
+        if (ctx->bricks && D == 3 && d.precision == ESPH_FP32_BULK && !thermal && !io && choose_bricks(hd, spacing, bd)) {
+            int64_t nb = 1;
+            for (int k = 0; k < 3; ++k) {
+                hd.bdim[k] = bd[k];
+                hd.nbk[k] = (hd.nc[k] + bd[k] - 1) / bd[k];
+                nb *= hd.nbk[k];
+            }
+            if (nb < ((int64_t)1 << 24)) {
+                hd.nbricks = (int32_t)nb;
+                brick_bytes = sizeof(int32_t) * (size_t)nb + brick_flag_bytes(hd.nbricks) +
+                              sizeof(uint16_t) * (size_t)cap * hd.kff;
+            } else {
+                hd.nbricks = 0;
+            }
+        }
+    }
+
     // size check against the context budget
     const bool f32 = d.precision == ESPH_FP32_BULK;
     const size_t F = sizeof(float), G = sizeof(double), I = sizeof(int32_t);
@@ -565,7 +626,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
                    sizeof(SceneDyn) + sizeof(SceneDev) +
                    ((size_t)(cap + kBlock - 1) / kBlock + (size_t)hd.cell_tiles) * sizeof(Tile) +
                    (size_t)cap * I * (hd.kff + hd.kfw + 4) + 2 * (size_t)cap * 4 * F + (f32 ? 2 * (size_t)cap * 4 * I : 0) +
-                   (size_t)cap * D * G;
+                   (size_t)cap * D * G + brick_bytes;
     if (ctx->budget && ctx->used + bytes > ctx->budget)
         return fail(ctx, ESPH_ERR_BUDGET, "scene needs " + std::to_string(bytes) + " bytes; budget left " +
                                               std::to_string(ctx->budget - ctx->used));
@@ -648,6 +709,12 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         hd.wkap = dal(P * std::max(nw, 1));
         hd.wxf = dal(F * (D == 3 ? 4 : 2) * std::max(nw, 1));
     }
+    if (hd.nbricks > 0) {
+        hd.blist = (int32_t*)dal(I * (size_t)hd.nbricks);
+        hd.bflag = (uint8_t*)dal(brick_flag_bytes(hd.nbricks));
+        hd.vslot = (uint16_t*)dal(sizeof(uint16_t) * (size_t)cap * hd.kff);
+        if (eos_walk_ok(s, f)) hd.flags |= FLAG_BRICK;
+    }
     hd.dyn = (SceneDyn*)dal(sizeof(SceneDyn));
     s->dev = (SceneDev*)dal(sizeof(SceneDev));
     s->nptiles = (cap + kBlock - 1) / kBlock;
@@ -661,8 +728,10 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     }
     ctx->used += a.total;
     hipStream_t st = ctx->stream;
-    // the sort expects zero cell counts (its scan clears them after use)
+    // the sort expects zero cell counts (its scan clears them after use),
+    // and no brick marks
     HIPCHK(ctx, hipMemsetAsync(hd.count, 0, I * (hd.ncell + 1), st));
+    if (hd.bflag) HIPCHK(ctx, hipMemsetAsync(hd.bflag, 0, brick_flag_bytes(hd.nbricks), st));
     // wall binning and upload
     {
         const WallHost& w = s->walls;
@@ -780,6 +849,8 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
             return fail(ctx, ESPH_ERR_STATE, "set_fluid: the new gaps change the scene's force walk while it is in a batch");
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         s->h.flags ^= FLAG_EOS_WALK;
+        // brick walks follow (their buffers exist when the scene allowed them)
+        if (s->h.nbricks > 0) s->h.flags = (s->h.flags & ~FLAG_BRICK) | ((s->h.flags & FLAG_EOS_WALK) ? FLAG_BRICK : 0);
         HIPCHK(ctx, hipMemcpy(s->dev, &s->h, sizeof(SceneDev), hipMemcpyHostToDevice));
         s->gplan.release();
     }
@@ -850,8 +921,13 @@ extern "C" int esph_scene_set_fluid(esph_scene* s, const esph_fluid_arrays* f) {
 static Bucket scene_bucket(esph_scene* s) {
     const bool th = (s->h.flags & FLAG_THERMAL) != 0;
     const bool eos = (s->h.flags & FLAG_EOS_WALK) != 0;
+    const bool br = (s->h.flags & FLAG_BRICK) != 0;
     Bucket b{s->d.dim, s->d.kernel, s->d.precision, s->dev, s->ptiles, s->nptiles, s->otiles, 1, th, !th && !eos};
-    b.has_eos = eos;
+    b.has_eos = eos && !br;
+    b.has_brick = br;
+    b.has_tile = !br;
+    b.btiles = s->ptiles;  // its first tile is (scene 0, particle 0)
+    b.nbtiles = br ? 1 : 0;
     return b;
 }
 
@@ -1290,6 +1366,20 @@ extern "C" int esph_scene_stats(esph_scene* s, int64_t* out8) {
     out6[5] = dyn.err;
     out8[6] = dyn.nsorts;
     out8[7] = s->h.verlet ? s->h.kff : 0;
+    return ESPH_OK;
+}
+
+extern "C" int esph_scene_brick_stats(esph_scene* s, int64_t* out6) {
+    if (!s || !out6) return ESPH_ERR_ARG;
+    if (!s->fluid_ready) return fail(s->ctx, ESPH_ERR_STATE, "no fluid uploaded");
+    SceneDyn dyn;
+    int rc = read_dyn(s, &dyn);
+    if (rc) return rc;
+    const bool br = (s->h.flags & FLAG_BRICK) != 0;
+    out6[0] = br ? 1 : 0;
+    for (int k = 0; k < 3; ++k) out6[1 + k] = br ? s->h.bdim[k] : 0;
+    out6[4] = br ? dyn.nbrick : 0;
+    out6[5] = br ? dyn.brick_spill : 0;
     return ESPH_OK;
 }
 

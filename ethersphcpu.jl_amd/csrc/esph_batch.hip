@@ -24,7 +24,7 @@ using namespace esph;
 namespace esph {
 
 std::array<int, 4> depth_key(const Bucket& b) {
-    return {b.dim, b.kernel, b.prec, (b.has_thermal ? 2 : 0) | (b.has_plain || b.has_eos ? 1 : 0)};
+    return {b.dim, b.kernel, b.prec, (b.has_thermal ? 2 : 0) | (b.has_plain || b.has_eos || b.has_brick ? 1 : 0)};
 }
 
 void apply_depth(const esph_ctx* ctx, Bucket& b) {
@@ -178,7 +178,7 @@ int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
     // per-bucket tile tables, laid out after the global ones in one buffer
     std::vector<Tile> tiles(all_p);
     tiles.insert(tiles.end(), all_c.begin(), all_c.end());
-    struct Off { size_t p, np, o, no; bool th, pl, eo; int dim, kernel, prec, ns; };
+    struct Off { size_t p, np, o, no, bk, nbk; bool th, pl, eo, br, tl; int dim, kernel, prec, ns; };
     std::vector<Off> offs;
     for (auto& g : groups) {
         Off o{};
@@ -191,15 +191,22 @@ int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
             for (int t = 0; t < members[k]->nptiles; ++t) tiles.push_back(Tile{k, t * kBlock});
             const bool th = (members[k]->h.flags & FLAG_THERMAL) != 0;
             const bool eo = (members[k]->h.flags & FLAG_EOS_WALK) != 0;
+            const bool br = (members[k]->h.flags & FLAG_BRICK) != 0;
             o.th = o.th || th;
             o.pl = o.pl || (!th && !eo);
-            o.eo = o.eo || eo;
+            o.eo = o.eo || (eo && !br);
+            o.br = o.br || br;
+            o.tl = o.tl || !br;
         }
         o.np = tiles.size() - o.p;
         o.o = tiles.size();
         for (int k : g.second)
             for (int t = 0; t < kOvfTiles; ++t) tiles.push_back(Tile{k, t * kBlock});
         o.no = tiles.size() - o.o;
+        o.bk = tiles.size();
+        for (int k : g.second)
+            if (members[k]->h.flags & FLAG_BRICK) tiles.push_back(Tile{k, 0});
+        o.nbk = tiles.size() - o.bk;
         offs.push_back(o);
     }
     SceneDev* dtable = (SceneDev*)arena.get(sizeof(SceneDev) * std::max(n, 1));
@@ -220,6 +227,10 @@ int StepPlan::build(esph_ctx* ctx, const std::vector<esph_scene*>& members) {
         buckets.push_back(Bucket{o.dim, o.kernel, o.prec, dtable, dtiles + o.p, (int)o.np, dtiles + o.o, o.ns, o.th,
                                  o.pl});
         buckets.back().has_eos = o.eo;
+        buckets.back().has_brick = o.br;
+        buckets.back().has_tile = o.tl;
+        buckets.back().btiles = dtiles + o.bk;
+        buckets.back().nbtiles = (int)o.nbk;
         apply_depth(ctx, buckets.back());
     }
     plan = SortPlan{dtable, n, dtiles, (int)all_p.size(), dtiles + all_p.size(), (int)all_c.size()};

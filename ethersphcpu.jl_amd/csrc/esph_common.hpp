@@ -92,7 +92,8 @@ __device__ __forceinline__ int4 ld_chunk(const int32_t* p) {
 // one index range [j0, j1).  Rows and row ends whose nearest point is
 // farther than R from xi (judged from xi's offset inside its cell, with a
 // slack that keeps the test conservative under rounding and clamping) are
-// skipped.  Calls row(j0, j1) for each remaining row.
+// skipped.  Calls row(j0, j1) -- or row(j0, j1, cy, cz) with the row's cell
+// coordinates -- for each remaining row.
 template <int D, typename F>
 __device__ __forceinline__ void for_each_row(const SceneDev& s, const int32_t* __restrict__ cstart,
                                              const double* xi, const int c[3], F&& row) {
@@ -137,7 +138,10 @@ __device__ __forceinline__ void for_each_row(const SceneDev& s, const int32_t* _
             const int cx0 = c[0] + xl > 0 ? c[0] + xl : 0;
             const int cx1 = c[0] + xh < s.nc[0] - 1 ? c[0] + xh : s.nc[0] - 1;
             if (cx0 > cx1) continue;
-            row(cstart[cell_linear(s, cx0, cy, cz)], cstart[cell_linear(s, cx1, cy, cz) + 1]);
+            if constexpr (std::is_invocable_v<F, int, int, int, int>)
+                row(cstart[cell_linear(s, cx0, cy, cz)], cstart[cell_linear(s, cx1, cy, cz) + 1], cy, cz);
+            else
+                row(cstart[cell_linear(s, cx0, cy, cz)], cstart[cell_linear(s, cx1, cy, cz) + 1]);
         }
     }
 }
@@ -180,12 +184,17 @@ __device__ __forceinline__ size_t list_at(size_t cap, int k, int i) {
 // there; only those within the rounding band tau of Rv take the FP64 rule
 // (d^2 <= Rv^2).  tau bounds the float error: 2^-21 (|xr_i|_inf + 2 Rv), so the
 // decisions equal the FP64 rule's.  Calls f(j) for every candidate, in cell
-// order.  Used right after a sort, when xf and the cells are current.
-template <int D, typename F>
+// order; hook(cy, cz) before the candidates of each cell row.  Used right
+// after a sort, when xf and the cells are current.
+struct NoRowHook {
+    __device__ __forceinline__ void operator()(int, int) const {}
+};
+
+template <int D, typename F, typename H = NoRowHook>
 __device__ __forceinline__ void for_each_candidate_f32(const SceneDev& s, const int32_t* __restrict__ cstart,
                                                        const void* __restrict__ xfall,
                                                        const double* __restrict__ xall, const double* xi,
-                                                       const int c[3], int self, F&& f) {
+                                                       const int c[3], int self, F&& f, H&& hook = H{}) {
     float xr[3] = {(float)(xi[0] - s.lo[0]), (float)(xi[1] - s.lo[1]), D == 3 ? (float)(xi[2] - s.lo[2]) : 0.0f};
     const double amax = fmax(fabs((double)xr[0]), fmax(fabs((double)xr[1]), fabs((double)xr[2])));
     const double tau = ldexp(amax + 2.0 * s.Rv, -21);
@@ -193,7 +202,8 @@ __device__ __forceinline__ void for_each_candidate_f32(const SceneDev& s, const 
     const float lo2 = rin > 0.0 ? (float)(rin * rin) * (1.0f - 0x1p-22f) : -1.0f;
     const float hi2 = (float)(rout * rout) * (1.0f + 0x1p-22f);
     const double R2 = s.Rv2;
-    for_each_row<D>(s, cstart, xi, c, [&](int j0, int j1) {
+    for_each_row<D>(s, cstart, xi, c, [&](int j0, int j1, int cy, int cz) {
+        hook(cy, cz);  // the candidates that follow lie in this row
         for (int j = j0; j < j1; j += 4) {
             float d2f[4];
 #pragma unroll
@@ -233,35 +243,59 @@ __device__ __forceinline__ void for_each_candidate_f32(const SceneDev& s, const 
 // the count.  Entries fill a 4-entry register buffer flushed as one int4.
 // Entries beyond `width` (a multiple of 4) are dropped and the particle
 // later scans its cells.
-template <int D>
+// With a brick map (BM not void: FLAG_BRICK fluid lists), each entry's slot
+// in the brick region's LDS staging also goes to s.vslot, in the same layout.
+struct NoBrickMap {};
+
+template <int D, typename BM = NoBrickMap>
 __device__ __forceinline__ int build_list(const SceneDev& s, const int32_t* __restrict__ cstart,
                                           const void* __restrict__ xfall, const double* __restrict__ xall,
                                           const double* xi, const int c[3], int i, int self,
-                                          int32_t* __restrict__ list, int width) {
+                                          int32_t* __restrict__ list, int width, BM* bm = nullptr) {
+    constexpr bool SLOTS = !std::is_same<BM, NoBrickMap>::value;
     int cnt = 0;
     int4 buf = make_int4(0, 0, 0, 0);
+    ushort4 sbuf = make_ushort4(0, 0, 0, 0);
     const size_t cap = (size_t)s.cap;
-    for_each_candidate_f32<D>(s, cstart, xfall, xall, xi, c, self, [&](int j) {
+    auto put = [&](int j) {
         const int r = cnt & 3;
         buf.x = r == 0 ? j : buf.x;
         buf.y = r == 1 ? j : buf.y;
         buf.z = r == 2 ? j : buf.z;
         buf.w = r == 3 ? j : buf.w;
-        if (r == 3 && cnt < width) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
+        if constexpr (SLOTS) {
+            const unsigned short q = (unsigned short)bm->slot(j);
+            sbuf.x = r == 0 ? q : sbuf.x;
+            sbuf.y = r == 1 ? q : sbuf.y;
+            sbuf.z = r == 2 ? q : sbuf.z;
+            sbuf.w = r == 3 ? q : sbuf.w;
+        }
+        if (r == 3 && cnt < width) {
+            *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
+            if constexpr (SLOTS) *reinterpret_cast<ushort4*>(s.vslot + list_at(cap, cnt & ~3, i)) = sbuf;
+        }
         ++cnt;
-    });
-    if ((cnt & 3) != 0 && cnt < width) *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
+    };
+    if constexpr (SLOTS)
+        for_each_candidate_f32<D>(s, cstart, xfall, xall, xi, c, self, put, [&](int cy, int cz) { bm->row(cy, cz); });
+    else
+        for_each_candidate_f32<D>(s, cstart, xfall, xall, xi, c, self, put);
+    if ((cnt & 3) != 0 && cnt < width) {
+        *reinterpret_cast<int4*>(list + list_at(cap, cnt & ~3, i)) = buf;
+        if constexpr (SLOTS) *reinterpret_cast<ushort4*>(s.vslot + list_at(cap, cnt & ~3, i)) = sbuf;
+    }
     return cnt;
 }
 
 // Rebuilds particle i's fluid and wall candidate lists from the cells (right
 // after a sort, or every step when the lists are not reused) and returns the
 // fluid candidate count.
-template <int D>
-__device__ __forceinline__ int rebuild_lists(const SceneDev& s, const FluidSet& S, const double* xi, int i) {
+template <int D, typename BM = NoBrickMap>
+__device__ __forceinline__ int rebuild_lists(const SceneDev& s, const FluidSet& S, const double* xi, int i,
+                                             BM* bm = nullptr) {
     int c[3];
     cell_of<D>(s, xi, c);
-    const int cnt = build_list<D>(s, s.cell_start, S.xf, S.x, xi, c, i, i, s.vnbr, s.kff);
+    const int cnt = build_list<D>(s, s.cell_start, S.xf, S.x, xi, c, i, i, s.vnbr, s.kff, bm);
     s.nvnb[i] = cnt;
     if (s.nw > 0) s.nnbw[i] = build_list<D>(s, s.wcell_start, s.wxf, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
     return cnt;
@@ -487,9 +521,10 @@ __device__ __forceinline__ float eos_pr2(const SceneDev& s, float d, float& rho)
 // float distance and one inside it -- or with a position beyond the
 // quantized range -- by the FP64 rule itself.  Returns whether j is a
 // neighbour; rv and d2 are the float geometry of the pair in quanta.
-template <int D>
-__device__ __forceinline__ bool qgeom(const SceneDev& s, const double* __restrict__ xall, const double* xi,
-                                      const int32_t* qi, const int32_t* qj, int j, float* rv, float& d2) {
+// (getj: returns j, called only inside the band)
+template <int D, typename J>
+__device__ __forceinline__ bool qgeom_f(const SceneDev& s, const double* __restrict__ xall, const double* xi,
+                                        const int32_t* qi, const int32_t* qj, J&& getj, float* rv, float& d2) {
 #pragma unroll
     for (int k = 0; k < D; ++k) rv[k] = (float)(int32_t)((uint32_t)qi[k] - (uint32_t)qj[k]);
     if (D == 2) rv[2] = 0.0f;
@@ -497,7 +532,7 @@ __device__ __forceinline__ bool qgeom(const SceneDev& s, const double* __restric
     bool in = d2 <= s.R2q;
     if (qi[0] == kQInvalid || qj[0] == kQInvalid || fabsf(d2 - s.R2q) <= s.qbandq) {
         double xj[3], rvd[3];
-        load_x<D>(xall, j, xj);
+        load_x<D>(xall, getj(), xj);
         const double d2d = dist2_exact<D>(xi, xj, rvd);
         in = d2d <= s.R2;
 #pragma unroll
@@ -505,6 +540,12 @@ __device__ __forceinline__ bool qgeom(const SceneDev& s, const double* __restric
         d2 = (float)(d2d * (s.inv_qu * s.inv_qu));
     }
     return in;
+}
+
+template <int D>
+__device__ __forceinline__ bool qgeom(const SceneDev& s, const double* __restrict__ xall, const double* xi,
+                                      const int32_t* qi, const int32_t* qj, int j, float* rv, float& d2) {
+    return qgeom_f<D>(s, xall, xi, qi, qj, [j] { return j; }, rv, d2);
 }
 
 // Length constants of the pair arithmetic in the units of its geometry:
@@ -605,6 +646,242 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
     }
 }
 
+// Velocity record of a neighbour (continuity pass).
+template <typename T>
+struct ARec {
+    T v[3];
+    T m, gap;
+};
+
+// Inclusive scan over a wavefront (64 lanes).
+__device__ __forceinline__ int wave_inclusive_scan(int x) {
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// Brick walk (FLAG_BRICK).  Bricks are blocks of bdim cells; a workgroup
+// takes one brick at a time, stages the quantized positions and A records of
+// every particle of the brick's region -- its cells widened by the stencil
+// on each side, i.e. every cell a neighbour of a brick particle can be in --
+// in LDS with contiguous loads (the region's cell rows along x are particle
+// ranges of the cell order), and walks its particles' candidate lists with
+// the candidates' data read from LDS.  The list rebuild records, beside each
+// candidate j, its LDS slot: the region's rows (fixed y, z) are staged in
+// row order, row r from slot rb[r], so slot = j - rs[r] + rb[r].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int brick_of_cell(const SceneDev& s, int c) {
+    const int cx = c % s.nc[0];
+    const int r = c / s.nc[0];
+    const int cy = r % s.nc[1];
+    const int cz = r / s.nc[1];
+    return cx / s.bdim[0] + s.nbk[0] * (cy / s.bdim[1] + s.nbk[1] * (cz / s.bdim[2]));
+}
+
+// Cell rows of one brick and of its region, in LDS.  Region row r (r < nr)
+// is cells [xa, xb] of row (y0 + r % ny, z0 + r / ny): particles
+// [rs[r], rs[r] + rb[r + 1] - rb[r]), staged from slot rb[r].  Brick row r
+// (r < no) is cells [bx0, bx1] of row (by0 + r % BY, bz0 + r / BY):
+// particles [os[r], os[r] + ob[r + 1] - ob[r]).
+struct BrickTables {
+    int rs[kBrickRows];
+    int rb[kBrickRows + 1];
+    int os[kBrickOwn];
+    int ob[kBrickOwn + 1];
+    int y0, z0, ny, nr, no;
+};
+
+// Slot map of a brick region for the list rebuild: row(cy, cz) before the
+// candidates of a cell row, slot(j) for each of them.
+struct BrickMap {
+    const BrickTables* t;
+    int base;
+    __device__ __forceinline__ void row(int cy, int cz) {
+        const int r = (cy - t->y0) + t->ny * (cz - t->z0);
+        base = t->rb[r] - t->rs[r];
+    }
+    __device__ __forceinline__ int slot(int j) const { return j + base; }
+};
+
+// Largest r in [0, n) with a[r] <= v (a non-decreasing, a[0] <= v).
+__device__ __forceinline__ int row_search(const int* a, int n, int v) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Fills the tables of brick b (all threads of the workgroup; ends with a
+// barrier).  Rows outside the grid are empty.
+__device__ __forceinline__ void brick_tables(const SceneDev& s, int b, BrickTables& t) {
+    const int BX = s.bdim[0], BY = s.bdim[1], BZ = s.bdim[2], m = s.stencil;
+    const int bx = b % s.nbk[0], b1 = b / s.nbk[0];
+    const int x0 = bx * BX, y0 = (b1 % s.nbk[1]) * BY, z0 = (b1 / s.nbk[1]) * BZ;
+    const int ny = BY + 2 * m, nr = ny * (BZ + 2 * m), no = BY * BZ;
+    const int32_t* cs = s.cell_start;
+    const int tid = (int)threadIdx.x;
+    if (tid < nr) {
+        const int cy = y0 - m + tid % ny, cz = z0 - m + tid / ny;
+        int st = 0, len = 0;
+        if (cy >= 0 && cy < s.nc[1] && cz >= 0 && cz < s.nc[2]) {
+            const int xa = x0 - m > 0 ? x0 - m : 0;
+            const int xb = x0 + BX - 1 + m < s.nc[0] - 1 ? x0 + BX - 1 + m : s.nc[0] - 1;
+            st = cs[cell_linear(s, xa, cy, cz)];
+            len = cs[cell_linear(s, xb, cy, cz) + 1] - st;
+        }
+        t.rs[tid] = st;
+        t.rb[tid + 1] = len;
+    } else if (tid >= kBrickRows && tid < kBrickRows + no) {
+        const int r = tid - kBrickRows;
+        const int cy = y0 + r % BY, cz = z0 + r / BY;
+        int st = 0, len = 0;
+        if (cy < s.nc[1] && cz < s.nc[2]) {
+            const int xb = x0 + BX - 1 < s.nc[0] - 1 ? x0 + BX - 1 : s.nc[0] - 1;
+            st = cs[cell_linear(s, x0, cy, cz)];
+            len = cs[cell_linear(s, xb, cy, cz) + 1] - st;
+        }
+        t.os[r] = st;
+        t.ob[r + 1] = len;
+    }
+    if (tid == 0) {
+        t.y0 = y0 - m;
+        t.z0 = z0 - m;
+        t.ny = ny;
+        t.nr = nr;
+        t.no = no;
+    }
+    __syncthreads();
+    // running sums: wave 0 the region's rows (two per lane), wave 1 the brick's
+    const int lane = tid & 63, wave = tid >> 6;
+    if (wave == 0) {
+        const int a = 2 * lane < nr ? t.rb[2 * lane + 1] : 0;
+        const int c = 2 * lane + 1 < nr ? t.rb[2 * lane + 2] : 0;
+        const int ex = wave_inclusive_scan(a + c) - (a + c);
+        if (2 * lane < nr) t.rb[2 * lane + 1] = ex + a;
+        if (2 * lane + 1 < nr) t.rb[2 * lane + 2] = ex + a + c;
+        if (lane == 0) t.rb[0] = 0;
+    } else if (wave == 1) {
+        const int a = lane < no ? t.ob[lane + 1] : 0;
+        const int in = wave_inclusive_scan(a);
+        if (lane < no) t.ob[lane + 1] = in;
+        if (lane == 0) t.ob[0] = 0;
+    }
+    __syncthreads();
+}
+
+// Stages the region of the current tables (t.rb[t.nr] particles, at most
+// kBrickCap) into lq / la; ends with a barrier.
+__device__ __forceinline__ void brick_stage(const FluidSet& S, const BrickTables& t, int4* lq, float4* la) {
+    const int total = t.rb[t.nr];
+    constexpr int U = 4;
+    for (int k0 = (int)threadIdx.x; k0 < total; k0 += U * kBrickThreads) {
+        int4 q[U];
+        float4 a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * kBrickThreads;
+            const int kk = k < total ? k : total - 1;
+            const int r = row_search(t.rb, t.nr, kk);
+            const int j = t.rs[r] + (kk - t.rb[r]);
+            q[u] = ldrec<int4>(S.xq, j);
+            a[u] = ldrec<float4>(S.A, j);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * kBrickThreads;
+            if (k < total) {
+                lq[k] = q[u];
+                la[k] = a[u];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Calls f(i) for each particle of the current brick (each thread its own).
+template <typename F>
+__device__ __forceinline__ void for_brick_particles(const BrickTables& t, F&& f) {
+    const int n = t.ob[t.no];
+    for (int q = (int)threadIdx.x; q < n; q += kBrickThreads) {
+        const int r = row_search(t.ob, t.no, q);
+        f(t.os[r] + (q - t.ob[r]));
+    }
+}
+
+// 4-entry chunk of LDS slots (FLAG_BRICK), loaded non-temporally.
+typedef unsigned short esph_nus4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ ushort4 ld_slots(const uint16_t* p) {
+    const esph_nus4 v = __builtin_nontemporal_load(reinterpret_cast<const esph_nus4*>(p));
+    return make_ushort4(v.x, v.y, v.z, v.w);
+}
+
+// The fluid candidates staged in LDS (FLAG_BRICK): quantized positions with
+// their w bits, and A records (velocity, mass), by slot.
+struct BrickStage {
+    const int4* q;
+    const float4* a;
+};
+
+// The list walk of for_pairs_q over a staged brick region: each candidate's
+// quantized position and A record come from its LDS slot (vslot) instead of
+// device memory, and its index (vnbr) is read only inside the rounding band,
+// where the FP64 rule needs its binary64 position.  Same candidates, order
+// and arithmetic as for_pairs_q: body(in, d2, rv, a, w).
+template <int DEPTH, typename Body>
+__device__ __forceinline__ void for_pairs_brick(const SceneDev& s, const FluidSet& S, const double* xi, int i,
+                                                int cnt, const BrickStage& st, Body&& body) {
+    static_assert(DEPTH == 1 || DEPTH == 2 || DEPTH == 4, "DEPTH divides a 4-entry chunk");
+    if (cnt <= 0) return;
+    int32_t qi[4];
+    load_q<3>(S.xq, i, qi);
+    const size_t cap = (size_t)s.cap;
+    const int nq = (cnt + 3) >> 2;
+    ushort4 cur = ld_slots(s.vslot + list_at(cap, 0, i));
+    for (int q = 0; q < nq; ++q) {
+        ushort4 nxt = cur;
+        if (q + 1 < nq) nxt = ld_slots(s.vslot + list_at(cap, 4 * (q + 1), i));
+        const int ks[4] = {cur.x, cur.y, cur.z, cur.w};
+        const int m = cnt - 4 * q;  // entries of this chunk: min(4, m)
+#pragma unroll
+        for (int g = 0; g < 4; g += DEPTH) {
+            if (g < m) {
+                int4 Q[DEPTH];
+                float4 A[DEPTH];
+#pragma unroll
+                for (int u = 0; u < DEPTH; ++u) {
+                    Q[u] = st.q[ks[g + u]];
+                    A[u] = st.a[ks[g + u]];
+                }
+#pragma unroll
+                for (int u = 0; u < DEPTH; ++u) {
+                    if (g + u < m) {
+                        const int k = 4 * q + g + u;
+                        const int32_t qj[4] = {Q[u].x, Q[u].y, Q[u].z, Q[u].w};
+                        float rv[3], d2;
+                        const bool in = qgeom_f<3>(s, S.x, xi, qi, qj, [&] { return s.vnbr[list_at(cap, k, i)]; },
+                                                   rv, d2);
+                        ARec<float> a;
+                        a.v[0] = A[u].x;
+                        a.v[1] = A[u].y;
+                        a.v[2] = A[u].z;
+                        a.m = A[u].w;
+                        a.gap = 0.0f;
+                        body(in, d2, rv, a, qj[3]);
+                    }
+                }
+            }
+        }
+        cur = nxt;
+    }
+}
+
 // The SCAN instances walk the step's overflow list with a small fixed grid:
 // kOvfTiles tiles of kBlock entries per scene, striding over the list.
 // Calls f(i) for the particle(s) of this thread: the tile slot (main
@@ -625,12 +902,6 @@ __device__ __forceinline__ bool overflowed(const SceneDev& s, int i) {
     return s.nvnb[i] > s.kff || (s.nw > 0 && s.nnbw[i] > s.kfw);
 }
 
-// Velocity record of a neighbour (continuity pass).
-template <typename T>
-struct ARec {
-    T v[3];
-    T m, gap;
-};
 
 // Unpacked neighbour record (see FluidSet).
 template <typename T>

@@ -22,6 +22,17 @@ constexpr int kCellTile = kBlock * kScanItems;
 constexpr int kOvfTiles = 16;        // tiles per scene of the overflow (cell-scan) kernels
 // waves per SIMD the force pass of FLAG_EOS_WALK scenes is compiled for
 constexpr int kEosWalkWaves = 4;
+// Brick walk (FLAG_BRICK): workgroups of kBrickThreads threads, each taking
+// one brick of cells at a time and staging the quantized positions and A
+// records of every particle of the brick's neighbour region (the brick
+// widened by the stencil on each side) in LDS, at most kBrickCap particles:
+// 2 x 16 bytes each, so two workgroups (sixteen waves) fit the 160 KB of LDS
+// of a CU.  kBrickRows / kBrickOwn bound the cell rows (fixed y, z) of the
+// region and of the brick.
+constexpr int kBrickThreads = 512;
+constexpr int kBrickCap = 2432;
+constexpr int kBrickRows = 128;
+constexpr int kBrickOwn = 64;
 
 // SceneDev::flags
 enum : int {
@@ -35,6 +46,10 @@ enum : int {
     // the equation of state (every fluid particle's p is EOS(rho) once the
     // continuity pass has run)
     FLAG_EOS_WALK = 16,
+    // FLAG_EOS_WALK scene whose continuity and force passes walk bricks of
+    // cells with their neighbour regions staged in LDS (SceneDev::bdim);
+    // candidates are addressed by their LDS slots (SceneDev::vslot)
+    FLAG_BRICK = 32,
 };
 
 // Particle kinds of the inlet/outlet solver (FluidSet::kind).
@@ -68,8 +83,16 @@ struct SceneDyn {
     int32_t scan_done;    // cell-count tiles scanned in this sort (reset by the last one)
     int32_t outside_acc;  // particles far outside the grid, counted by the running sort
     int32_t shepard_ran;  // 1: this step's Shepard pass wrote rhon (the sort moves it into rho)
+    int32_t nbrick;       // FLAG_BRICK: bricks holding particles (SceneDev::blist), set by each sort
+    int32_t brick_spill;  // FLAG_BRICK: brick walks whose region exceeded kBrickCap (walked from device memory)
     int32_t pad_dyn;
 };
+
+// Bytes of SceneDev::bflag for nb bricks: whole 16-byte words, kBlock per
+// 4,096 bricks (the compaction gives each thread of one block a run of words).
+__host__ __device__ inline size_t brick_flag_bytes(int nb) {
+    return ((size_t)(nb + 16 * kBlock - 1) / (16 * kBlock)) * (16 * kBlock);
+}
 
 // Index into SceneDev::set of the current state and of the step's outputs
 // (same particle order, the other per-step copy).
@@ -208,6 +231,18 @@ struct SceneDev {
     int32_t io_axis, profile, profile_terms, outlet_profile;
     double inlet_enter, inlet_shift, outlet_enter, outlet_exit;
     double profile_f, profile_nu, profile_l, profile_c[3];
+    // bricks (FLAG_BRICK): bdim[k] cells per brick along axis k, nbk[k]
+    // bricks along it; brick (bx, by, bz) is bx + nbk[0] (by + nbk[1] bz)
+    int32_t bdim[3];
+    int32_t nbk[3];
+    int32_t nbricks;
+    int32_t pad7;
+    int32_t* blist;   // bricks holding particles, in brick order (SceneDyn::nbrick of them)
+    uint8_t* bflag;   // nbricks marks of the running sort (cleared when blist is written)
+    // LDS slot of every fluid candidate within its particle's brick region,
+    // in the chunk layout of vnbr (entry k of particle i at
+    // [((k / 4) * cap + i) * 4 + k % 4]); written with vnbr at each rebuild
+    uint16_t* vslot;
     int32_t* spawn_src;   // per spawn (append order): source inlet particle
     int32_t* spawn_id;    //   its inlet sequence number
     int32_t* spawn_key;   //   cell of the copy

@@ -118,6 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
             }
         }
         key = dead ? s.ncell : cell_key(s, D, x);
+        if (!dead && (s.flags & FLAG_BRICK)) s.bflag[brick_of_cell(s, key)] = 1;
         if (!finite) atomicOr(&s.dyn->err, 1);
         {
             // particles beyond the grid share its border cells (clamped):
@@ -143,17 +144,6 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
             }
         }
     }
-}
-
-// Inclusive scan over a wavefront (64 lanes).
-__device__ __forceinline__ int wave_inclusive_scan(int x) {
-    const int lane = (int)(threadIdx.x & 63);
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    return x;
 }
 
 // Block-wide exclusive scan of one value per thread; returns the exclusive
@@ -186,6 +176,33 @@ __device__ __forceinline__ void scan_tile_totals(const SceneDev& s) {
         if (q < s.cell_tiles) s.bsum[q] = carry + ex;
         carry += total;
     }
+}
+
+// The bricks marked by this sort's binning, in brick order, into blist
+// (FLAG_BRICK), and the marks cleared; by one block.  Each thread takes a
+// contiguous run of 16-byte words of the marks (one byte per brick, 0 or 1,
+// the array padded with zeros to a multiple of 16 * kBlock bytes).
+__device__ __forceinline__ void compact_bricks(const SceneDev& s) {
+    const int nw = ((s.nbricks + 16 * kBlock - 1) / (16 * kBlock)) * kBlock;  // 16-byte words
+    const int per = nw / kBlock;
+    uint4* w = reinterpret_cast<uint4*>(s.bflag) + (size_t)threadIdx.x * per;
+    int c = 0;
+    for (int q = 0; q < per; ++q) {
+        const uint4 v = w[q];
+        c += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+    }
+    int total;
+    int off = block_exclusive_scan(c, &total);
+    const int b0 = (int)threadIdx.x * per * 16;
+    for (int q = 0; q < per; ++q) {
+        const uint4 v = w[q];
+        const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if ((u[k >> 2] >> (8 * (k & 3))) & 0xffu) s.blist[off++] = b0 + 16 * q + k;
+        w[q] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    if (threadIdx.x == 0) s.dyn->nbrick = total;
 }
 
 // Phases 1 and 2: scan of kCellTile counts per tile, tile total to bsum; the
@@ -224,6 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
         if (last) {
             __threadfence();
             scan_tile_totals(s);
+            if (s.flags & FLAG_BRICK) compact_bricks(s);
             if (threadIdx.x == 0) s.dyn->scan_done = 0;
         }
         __syncthreads();

@@ -20,7 +20,11 @@ struct Bucket {
     int nscenes;
     bool has_thermal;        // some scene of the bucket is thermal
     bool has_plain;          // some scene of the bucket is neither thermal nor FLAG_EOS_WALK
-    bool has_eos = false;    // some scene of the bucket has FLAG_EOS_WALK
+    bool has_eos = false;    // some scene of the bucket has FLAG_EOS_WALK but not FLAG_BRICK
+    bool has_brick = false;  // some scene of the bucket has FLAG_BRICK (brick walks)
+    bool has_tile = true;    // some scene of the bucket has not (tile walks)
+    const Tile* btiles = nullptr;  // device: one entry (scene, 0) per FLAG_BRICK scene
+    int nbtiles = 0;
     int depth = 2;           // gather depth of the force pass (1, 2 or 4; see the tuner)
     int bs = 256;            // workgroup size of the force pass (64, 128 or 256; see the tuner)
     int cdepth = 2, cbs = 256;  // the same for the continuity pass

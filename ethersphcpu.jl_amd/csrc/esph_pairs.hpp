@@ -18,12 +18,125 @@ namespace esph {
 // sums equal those over the neighbours alone, and the wavefront stays
 // convergent.
 // ---------------------------------------------------------------------------
-// dry (tuner runs): the step's lists as they are, overflowed particles left
-// out, the new density to SceneDev::rhon only.
+// One particle of the continuity pass.  dry (tuner runs): the step's lists
+// as they are, overflowed particles left out, the new density to
+// SceneDev::rhon only.  BRICK (FLAG_BRICK scenes, 3D FP32_BULK): a rebuild
+// also records the candidates' LDS slots in the brick region (bm), and the
+// walk reads the candidates from the staged region (bst; null when the
+// region exceeded kBrickCap: from device memory, as the other scenes).
+template <int D, int K, typename T, bool SCAN, int DEPTH, bool BRICK = false>
+__device__ __forceinline__ void continuity_particle(const SceneDev& s, const SceneDyn& dyn, int i, int dry,
+                                                    BrickMap* bm = nullptr, const BrickStage* bst = nullptr) {
+    using KQ = KernelQ<T, K>;
+    const FluidSet& S = s.set[cur_set(dyn)];
+    if (S.kind && S.kind[i] != KIND_FLUID) {
+        // inlet/outlet buffers keep rho and p (drho is reset,
+        // FluidInletOutletWallSolve.jl:91-99) and need no lists
+        if (dry) return;
+        s.nvnb[i] = 0;
+        if (s.nw > 0) s.nnbw[i] = 0;
+        return;
+    }
+    const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
+    double xi[3];
+    load_x<D>(S.x, i, xi);
+    T vi[3], m_i, gap_i;
+    load_A<D, T>(S.A, i, vi, m_i, gap_i);
+    int cnt = 0;
+    if (!SCAN) {
+        if ((dyn.rebuild || !s.verlet) && !dry) {
+            // right after a sort: candidate lists within Rv from the cells
+            if constexpr (BRICK)
+                cnt = rebuild_lists<D>(s, S, xi, i, bm);
+            else
+                cnt = rebuild_lists<D>(s, S, xi, i);
+        } else {
+            cnt = s.nvnb[i];
+        }
+        if (dry && overflowed(s, i)) return;
+        if (overflowed(s, i)) {
+            // finished by the SCAN instance of this kernel
+            s.ovf[atomicAdd(&s.dyn->novf, 1)] = i;
+            atomicAdd(&s.dyn->overflow, 1);
+            return;
+        }
+    }
+    // candidates beyond R (the lists hold every particle within Rv)
+    // contribute exactly zero: weighted by 0 instead of branched around
+    const double R2 = s.R2;
+    T drho = T(0);
+    auto load = [&](int j) {
+        ARec<T> a;
+        load_A<D, T>(S.A, j, a.v, a.m, a.gap);
+        return a;
+    };
+    // one candidate: in = 1 for a neighbour under the exact rule, else 0;
+    // ih = 1 / h in the units of d2 and rv (QScale)
+    auto pair = [&](T ih, T in, T d2, const T* rv, const ARec<T>& a) {
+        const T* vj = a.v;
+        const T m_j = a.m;
+        const T r = psqrt(d2);
+        T W, dW;
+        KQ::wdw(r * ih, kA, kB, W, dW);
+        // normalize(0) is NaN and the reference zeroes it (Neighbour.jl:54-59)
+        const T sg = d2 == T(0) ? T(0) : dW * prcp(r);
+        T dot = T(0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) dot += (vi[k] - vj[k]) * rv[k];
+        drho += m_j * sg * dot * in;
+    };
+    if constexpr (sizeof(T) == 4 && !SCAN) {
+        // FP32_BULK: geometry from the quantized positions (FP64 rule near R)
+        const T ihq = QScale<T>(s, true).inv_h;
+        auto qbody = [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t) {
+            pair(ihq, in ? T(1) : T(0), d2, rv, a);
+        };
+        if constexpr (BRICK) {
+            if (bst) {
+                for_pairs_brick<DEPTH>(s, S, xi, i, cnt, *bst,
+                                       [&](bool in, float d2, const float* rv, const ARec<float>& a, int32_t w) {
+                    qbody(0, in, d2, rv, a, w);
+                });
+            } else {
+                // a region beyond the staging (rare): one gather at a time,
+                // which keeps the kernel's registers to the brick walk's
+                for_pairs_q<D, 1, false>(s, S, xi, i, s.vnbr, cnt, load, qbody);
+            }
+        } else {
+            for_pairs_q<D, DEPTH>(s, S, xi, i, s.vnbr, cnt, load, qbody);
+        }
+    } else {
+        for_pairs_d<D, SCAN, DEPTH, true, D == 3>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
+                                    [&](int, double d2, const double* rv, const ARec<T>& a) {
+            T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
+            pair(inv_h, d2 <= R2 ? T(1) : T(0), (T)d2, rvt, a);
+        });
+    }
+    {
+#pragma clang fp contract(off)
+        double rho = S.rho[i];
+        rho += (double)drho * s.dt;
+        if (dry) {
+            s.rhon[i] = rho;
+            return;
+        }
+        const double rr = rho / s.rho0;
+        const double p = s.b * (ipow(rr, s.gamma) - 1.0) + s.p0;
+        const double cc = s.c0 * sqrt(ipow(rr, s.gamma - 1));
+        S.rho[i] = rho;
+        S.p[i] = p;
+        S.c[i] = cc;
+        T* B = static_cast<T*>(S.B) + 4 * (size_t)i;
+        B[0] = (T)rho;
+        B[1] = (T)(p / (rho * rho));
+        // the density the force pass's two-record walk reads (FluidSet::xq)
+        if (D == 3 && S.xq) reinterpret_cast<float*>(S.xq)[4 * (size_t)i + 3] = (float)(rho - s.rho0);
+    }
+}
+
 template <int D, int K, typename T, bool SCAN, int DEPTH, int BS>
 __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles, int ntiles, int dry) {
-    using KQ = KernelQ<T, K>;
     constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
     const int nv = ntiles * SUB;
     for (int vb = blockIdx.x; vb < nv; vb += gridDim.x) {
@@ -31,98 +144,58 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_continuity(const SceneDe
         const Tile tile = tiles[v / SUB];
         const int first = tile.begin + (v % SUB) * BS;
         const SceneDev& s = scenes[tile.scene];
+        if (!SCAN && (s.flags & FLAG_BRICK)) continue;  // k_continuity_brick
         const SceneDyn dyn = *s.dyn;
-        for_work<SCAN>(s, dyn, tile, first, [&](int i) {
-        const FluidSet& S = s.set[cur_set(dyn)];
-        if (S.kind && S.kind[i] != KIND_FLUID) {
-            // inlet/outlet buffers keep rho and p (drho is reset,
-            // FluidInletOutletWallSolve.jl:91-99) and need no lists
-            if (dry) return;
-            s.nvnb[i] = 0;
-            if (s.nw > 0) s.nnbw[i] = 0;
-            return;
-        }
-        const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
-        double xi[3];
-        load_x<D>(S.x, i, xi);
-        T vi[3], m_i, gap_i;
-        load_A<D, T>(S.A, i, vi, m_i, gap_i);
-        int cnt = 0;
-        if (!SCAN) {
-            if ((dyn.rebuild || !s.verlet) && !dry) {
-                // right after a sort: candidate lists within Rv from the cells
-                cnt = rebuild_lists<D>(s, S, xi, i);
-            } else {
-                cnt = s.nvnb[i];
-            }
-            if (dry && overflowed(s, i)) return;
-            if (overflowed(s, i)) {
-                // finished by the SCAN instance of this kernel
-                s.ovf[atomicAdd(&s.dyn->novf, 1)] = i;
-                atomicAdd(&s.dyn->overflow, 1);
-                return;
-            }
-        }
-        // candidates beyond R (the lists hold every particle within Rv)
-        // contribute exactly zero: weighted by 0 instead of branched around
-        const double R2 = s.R2;
-        T drho = T(0);
-        auto load = [&](int j) {
-            ARec<T> a;
-            load_A<D, T>(S.A, j, a.v, a.m, a.gap);
-            return a;
-        };
-        // one candidate: in = 1 for a neighbour under the exact rule, else 0;
-        // ih = 1 / h in the units of d2 and rv (QScale)
-        auto pair = [&](T ih, T in, T d2, const T* rv, const ARec<T>& a) {
-            const T* vj = a.v;
-            const T m_j = a.m;
-            const T r = psqrt(d2);
-            T W, dW;
-            KQ::wdw(r * ih, kA, kB, W, dW);
-            // normalize(0) is NaN and the reference zeroes it (Neighbour.jl:54-59)
-            const T sg = d2 == T(0) ? T(0) : dW * prcp(r);
-            T dot = T(0);
-#pragma unroll
-            for (int k = 0; k < D; ++k) dot += (vi[k] - vj[k]) * rv[k];
-            drho += m_j * sg * dot * in;
-        };
-        if constexpr (sizeof(T) == 4 && !SCAN) {
-            // FP32_BULK: geometry from the quantized positions (FP64 rule near R)
-            const T ihq = QScale<T>(s, true).inv_h;
-            for_pairs_q<D, DEPTH>(s, S, xi, i, s.vnbr, cnt, load,
-                                  [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t) {
-                pair(ihq, in ? T(1) : T(0), d2, rv, a);
-            });
-        } else {
-            for_pairs_d<D, SCAN, DEPTH, true, D == 3>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
-                                        [&](int, double d2, const double* rv, const ARec<T>& a) {
-                T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
-                pair(inv_h, d2 <= R2 ? T(1) : T(0), (T)d2, rvt, a);
-            });
-        }
-        {
-#pragma clang fp contract(off)
-            double rho = S.rho[i];
-            rho += (double)drho * s.dt;
-            if (dry) {
-                s.rhon[i] = rho;
-                return;
-            }
-            const double rr = rho / s.rho0;
-            const double p = s.b * (ipow(rr, s.gamma) - 1.0) + s.p0;
-            const double cc = s.c0 * sqrt(ipow(rr, s.gamma - 1));
-            S.rho[i] = rho;
-            S.p[i] = p;
-            S.c[i] = cc;
-            T* B = static_cast<T*>(S.B) + 4 * (size_t)i;
-            B[0] = (T)rho;
-            B[1] = (T)(p / (rho * rho));
-            // the density the force pass's two-record walk reads (FluidSet::xq)
-            if (D == 3 && S.xq) reinterpret_cast<float*>(S.xq)[4 * (size_t)i + 3] = (float)(rho - s.rho0);
-        }
-        });
+        for_work<SCAN>(s, dyn, tile, first, [&](int i) { continuity_particle<D, K, T, SCAN, DEPTH>(s, dyn, i, dry); });
     }
+}
+
+// Bricks of the FLAG_BRICK scenes listed in bt (one entry per scene), XCD by
+// XCD: the scene's bricks (blist, in brick order) are cut into eight
+// contiguous ranges, and workgroup g (on XCD g % 8) takes entries g / 8,
+// g / 8 + gridDim.x / 8, ... of range g % 8, so each XCD's L2 serves one
+// compact part of the scene.  For each: the region's tables, its staging
+// (unless it exceeds kBrickCap), then f(s, dyn, spill) with every thread.
+template <typename F>
+__device__ __forceinline__ void for_each_brick(const SceneDev* __restrict__ scenes, const Tile* __restrict__ bt,
+                                               int nbt, BrickTables& tb, int4* lq, float4* la, F&& f) {
+    for (int e = 0; e < nbt; ++e) {
+        const SceneDev& s = scenes[bt[e].scene];
+        if (!(s.flags & FLAG_BRICK)) continue;
+        const SceneDyn dyn = *s.dyn;
+        const FluidSet& S = s.set[cur_set(dyn)];
+        const int nb = dyn.nbrick;
+        const int per = (nb + 7) >> 3;
+        const int xg = (int)(blockIdx.x & 7u), g8 = (int)(gridDim.x >> 3);
+        for (int k = (int)(blockIdx.x >> 3); k < per; k += g8) {
+            const int q = xg * per + k;
+            if (q >= nb) break;
+            brick_tables(s, s.blist[q], tb);
+            const bool spill = tb.rb[tb.nr] > kBrickCap;
+            if (!spill)
+                brick_stage(S, tb, lq, la);
+            else if (threadIdx.x == 0)
+                atomicAdd(&s.dyn->brick_spill, 1);
+            f(s, dyn, spill);
+            __syncthreads();  // the next brick rewrites the tables and the staging
+        }
+    }
+}
+
+// The continuity pass of FLAG_BRICK scenes (3D FP32_BULK), brick by brick.
+template <int K, int DEPTH>
+__global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_continuity_brick(
+    const SceneDev* __restrict__ scenes, const Tile* __restrict__ bt, int nbt, int dry) {
+    __shared__ int4 lq[kBrickCap];
+    __shared__ float4 la[kBrickCap];
+    __shared__ BrickTables tb;
+    for_each_brick(scenes, bt, nbt, tb, lq, la, [&](const SceneDev& s, const SceneDyn& dyn, bool spill) {
+        BrickMap bm{&tb, 0};
+        const BrickStage st{lq, la};
+        for_brick_particles(tb, [&](int i) {
+            continuity_particle<3, K, float, false, DEPTH, true>(s, dyn, i, dry, &bm, spill ? nullptr : &st);
+        });
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -184,9 +257,12 @@ __device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S
 // particle order (the first step after a sort), so B is written whole;
 // otherwise B.zw there still hold this order's gap (and, without heat, T) and
 // B.xy is rewritten by the next continuity pass before any read.
-template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH, bool EOSW>
+// BRICK (FLAG_BRICK scenes): the fluid walk reads the candidates from the
+// staged brick region bst (null when the region exceeded kBrickCap: from
+// device memory, as the other scenes).
+template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH, bool EOSW, bool BRICK = false>
 __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet& S, const FluidSet& N, int i,
-                                              bool full_records) {
+                                              bool full_records, const BrickStage* bst = nullptr) {
     using KQ = KernelQ<T, K>;
     if (S.kind && S.kind[i] != KIND_FLUID) {
         buffer_move<D, T>(s, S, N, i);
@@ -254,8 +330,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 load_A<D, T>(S.A, j, a.v, a.m, a.gap);
                 return a;
             };
-            for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA,
-                                         [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t w) {
+            auto body = [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t w) {
                 Rec<T> o;
                 o.v[0] = a.v[0];
                 o.v[1] = a.v[1];
@@ -263,7 +338,18 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 o.m = a.m;
                 o.pr2 = eos_pr2(s, __int_as_float(w), o.rho);
                 ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::true_type{}, gq);
-            });
+            };
+            if constexpr (BRICK) {
+                if (bst)
+                    for_pairs_brick<DEPTH>(s, S, xi, i, s.nvnb[i], *bst,
+                                           [&](bool in, float d2, const float* rv, const ARec<float>& a, int32_t w) {
+                        body(0, in, d2, rv, a, w);
+                    });
+                else  // a region beyond the staging (rare): one gather at a time
+                    for_pairs_q<D, 1, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA, body);
+            } else {
+                for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA, body);
+            }
         }
     } else if constexpr (sizeof(T) == 4 && !SCAN && D == 3) {
         // FP32_BULK in 3D: geometry from the quantized positions (FP64 rule
@@ -307,7 +393,10 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
             return w;
         };
         const double R2 = s.R2;
-        for_pairs_d<D, SCAN, DEPTH, SPLIT>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], wload,
+        // (brick walks: at most two wall gathers in flight, which keeps the
+        // four-deep fluid walk within the registers of four waves per SIMD)
+        constexpr int WDEPTH = BRICK && DEPTH > 2 ? 2 : DEPTH;
+        for_pairs_d<D, SCAN, WDEPTH, SPLIT>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], wload,
                                     [&](int j, double d2, const double* rv, const WRec& w) {
             // candidates within Rv: the exact rule selects the terms (no branch)
             const bool in = d2 <= R2;
@@ -431,6 +520,23 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     }
 }
 
+// Wavefront maximum of the squared displacements (rounded up to float;
+// float bits of non-negative values and NaN order as unsigned integers),
+// then one atomic per wavefront only when it raises the scene's running
+// maximum: a contended atomic per wavefront would serialise the whole pass
+// on one address.  Every lane of the wavefront calls it.
+__device__ __forceinline__ void record_disp(const SceneDev& s, double dmax) {
+    uint32_t bits = __float_as_uint(__double2float_ru(dmax));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t other = __shfl_xor(bits, o, 64);
+        bits = other > bits ? other : bits;
+    }
+    if ((threadIdx.x & 63) == 0 && bits != 0 &&
+        bits > __hip_atomic_load(&s.dyn->disp2_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&s.dyn->disp2_bits, bits);
+}
+
 // Float forces without heat are held to 4 waves per SIMD (128 VGPRs): the
 // gather pipeline otherwise takes a few registers more and falls to 3.
 // EOSW: the instance for scenes with FLAG_EOS_WALK (3D FP32_BULK without
@@ -451,6 +557,7 @@ __attribute__((amdgpu_waves_per_eu(EOSW && !SCAN ? kEosWalkWaves
         const SceneDyn dyn = *s.dyn;
         if (((s.flags & FLAG_THERMAL) != 0) != THERMAL) continue;
         if (((s.flags & FLAG_EOS_WALK) != 0) != EOSW) continue;
+        if (!SCAN && (s.flags & FLAG_BRICK)) continue;  // k_forces_brick
         double dmax = 0.0;
         for_work<SCAN>(s, dyn, tile, first, [&](int i) {
             if (!SCAN && overflowed(s, i)) return;
@@ -459,23 +566,29 @@ __attribute__((amdgpu_waves_per_eu(EOSW && !SCAN ? kEosWalkWaves
                                                                  dyn.rebuild != 0);
             dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
         });
-        if (s.verlet) {
-            // wavefront maximum of the squared displacements (rounded up to
-            // float; float bits of non-negative values and NaN order as
-            // unsigned integers), then one atomic per wavefront only when it
-            // raises the scene's running maximum: a contended atomic per
-            // wavefront would serialise the whole pass on one address
-            uint32_t bits = __float_as_uint(__double2float_ru(dmax));
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const uint32_t other = __shfl_xor(bits, o, 64);
-                bits = other > bits ? other : bits;
-            }
-            if ((threadIdx.x & 63) == 0 && bits != 0 &&
-                bits > __hip_atomic_load(&s.dyn->disp2_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                atomicMax(&s.dyn->disp2_bits, bits);
-        }
+        if (s.verlet) record_disp(s, dmax);
     }
+}
+
+// The force pass of FLAG_BRICK scenes (3D FP32_BULK, FLAG_EOS_WALK), brick
+// by brick.
+template <int K, int DEPTH>
+__global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_forces_brick(
+    const SceneDev* __restrict__ scenes, const Tile* __restrict__ bt, int nbt) {
+    __shared__ int4 lq[kBrickCap];
+    __shared__ float4 la[kBrickCap];
+    __shared__ BrickTables tb;
+    for_each_brick(scenes, bt, nbt, tb, lq, la, [&](const SceneDev& s, const SceneDyn& dyn, bool spill) {
+        const BrickStage st{lq, la};
+        double dmax = 0.0;
+        for_brick_particles(tb, [&](int i) {
+            if (overflowed(s, i)) return;
+            const double d2 = forces_body<3, K, float, false, false, DEPTH, true, true>(
+                s, s.set[cur_set(dyn)], s.set[next_set(dyn)], i, dyn.rebuild != 0, spill ? nullptr : &st);
+            dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
+        });
+        if (s.verlet) record_disp(s, dmax);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -560,12 +673,18 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* 
         }                                                                                       \
     } while (0)
 
+// Workgroups of the brick kernels: eight ranges of a scene's bricks, one per
+// XCD, kBrickGrid / 8 workgroups each (each workgroup one brick per scene
+// while a scene has at most kBrickGrid bricks holding particles).
+constexpr int kBrickGrid = 4096;
+
 template <int D>
 hipError_t launch_continuity_d(const Bucket& b, hipStream_t st) {
     hipError_t e = hipSuccess;
     const int bs_ = b.cbs, dry = b.tuning ? 1 : 0;
+    if (b.ntiles <= 0) return e;
     ESPH_DISPATCH_K(b.kernel, {
-        e = launch_both(b, st, [&](auto scan, const Tile* tl, int nt) {
+        auto launch = [&](auto scan, const Tile* tl, int nt) {
             constexpr bool S = decltype(scan)::value;
             ESPH_DISPATCH_DEPTH(S ? 1 : b.cdepth, {
                 if (b.prec == ESPH_FP64)
@@ -573,18 +692,31 @@ hipError_t launch_continuity_d(const Bucket& b, hipStream_t st) {
                 else
                     ESPH_PAIR_LAUNCH(SINGLE_ARG(k_continuity<D, K, float, S, DP, BSZ>), b.scenes, tl, nt, dry);
             });
-        });
+        };
+        if (b.has_tile) launch(std::false_type{}, b.tiles, b.ntiles);
+        if constexpr (D == 3) {
+            if (b.has_brick && b.prec == ESPH_FP32_BULK) {
+                ESPH_DISPATCH_DEPTH(b.cdepth, {
+                    hipLaunchKernelGGL((k_continuity_brick<K, DP>), dim3(kBrickGrid), dim3(kBrickThreads), 0, st,
+                                       b.scenes, b.btiles, b.nbtiles, dry);
+                });
+            }
+        }
+        // the overflowed particles, listed by the passes above
+        if (!b.tuning) launch(std::true_type{}, b.ovf_tiles, kOvfTiles * b.nscenes);
+        e = hipGetLastError();
     });
     return e;
 }
 
 template <int D, bool TH, bool EW = false>
-hipError_t launch_forces_t(const Bucket& b, hipStream_t st) {
+hipError_t launch_forces_t(const Bucket& b, hipStream_t st, bool main = true) {
     hipError_t e = hipSuccess;
     const int bs_ = b.bs;
     ESPH_DISPATCH_K(b.kernel, {
         e = launch_both(b, st, [&](auto scan, const Tile* tl, int nt) {
             constexpr bool S = decltype(scan)::value;
+            if (!S && !main) return;
             ESPH_DISPATCH_DEPTH(S ? 1 : b.depth, {
                 if constexpr (EW)
                     ESPH_PAIR_LAUNCH(SINGLE_ARG(k_forces<D, K, float, TH, S, DP, BSZ, true>), b.scenes, tl, nt);
@@ -599,14 +731,28 @@ hipError_t launch_forces_t(const Bucket& b, hipStream_t st) {
 }
 
 // One force-pass instance per kind of scene in the bucket: thermal, plain,
-// and (3D FP32_BULK) the two-record walk of FLAG_EOS_WALK scenes.
+// (3D FP32_BULK) the two-record walk of FLAG_EOS_WALK scenes, and their brick
+// walk (FLAG_BRICK; the two-record instance then runs for their overflowed
+// particles only, unless other scenes need it).
 template <int D>
 hipError_t launch_forces_d(const Bucket& b, hipStream_t st) {
     hipError_t e = hipSuccess;
     if (b.has_thermal) e = launch_forces_t<D, true>(b, st);
     if (e == hipSuccess && b.has_plain) e = launch_forces_t<D, false>(b, st);
-    if constexpr (D == 3)
-        if (e == hipSuccess && b.has_eos && b.prec == ESPH_FP32_BULK) e = launch_forces_t<D, false, true>(b, st);
+    if constexpr (D == 3) {
+        if (b.prec == ESPH_FP32_BULK) {
+            if (e == hipSuccess && (b.has_eos || b.has_brick)) e = launch_forces_t<D, false, true>(b, st, b.has_eos);
+            if (e == hipSuccess && b.has_brick && b.ntiles > 0) {
+                ESPH_DISPATCH_K(b.kernel, {
+                    ESPH_DISPATCH_DEPTH(b.depth, {
+                        hipLaunchKernelGGL((k_forces_brick<K, DP>), dim3(kBrickGrid), dim3(kBrickThreads), 0, st,
+                                           b.scenes, b.btiles, b.nbtiles);
+                    });
+                });
+                e = hipGetLastError();
+            }
+        }
+    }
     return e;
 }
 

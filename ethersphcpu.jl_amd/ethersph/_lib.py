@@ -96,6 +96,7 @@ def lib() -> ctypes.CDLL:
                                  POINTER(c_float)], c_int32),
         "esph_ctx_set_tuning": ([vp, c_int32, c_int32], c_int32),
         "esph_ctx_set_workgroup": ([vp, c_int32], c_int32),
+        "esph_ctx_set_brick_walk": ([vp, c_int32], c_int32),
         "esph_ctx_tuned_workgroup": ([vp, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_float)],
                                      c_int32),
         "esph_ctx_tuned_depth": ([vp, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_float)],
@@ -122,6 +123,7 @@ def lib() -> ctypes.CDLL:
         "esph_scene_enable_timing": ([vp, c_int32], c_int32),
         "esph_scene_phase_times": ([vp, DP, POINTER(c_int64)], c_int32),
         "esph_scene_stats": ([vp, POINTER(c_int64)], c_int32),
+        "esph_scene_brick_stats": ([vp, POINTER(c_int64)], c_int32),
         "esph_scene_enable_graph": ([vp, c_int32], c_int32),
         "esph_batch_create": ([vp, c_int32, POINTER(vp), POINTER(vp)], c_int32),
         "esph_batch_destroy": ([vp], c_int32),
@@ -188,6 +190,12 @@ class Context:
         """Workgroup size of the pair passes (64, 128 or 256 threads) when
         tuning is off."""
         self.check(self._L.esph_ctx_set_workgroup(self.handle, threads))
+
+    def set_brick_walk(self, enable: bool = True) -> None:
+        """Brick walks (LDS-staged neighbour regions) for the 3D FP32_BULK
+        scenes that allow them (default off); applies to scenes uploaded
+        afterwards.  Results are identical either way."""
+        self.check(self._L.esph_ctx_set_brick_walk(self.handle, 1 if enable else 0))
 
     def tuned_workgroup(self, dim: int, kernel: int, precision: int, thermal_mix: int = 1):
         """(threads, [ms at 64, 128, 256 threads]) of a tuned class, or None."""

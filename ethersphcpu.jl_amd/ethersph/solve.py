@@ -345,6 +345,14 @@ class Scene:
         return {"list_overflow": v[0], "list_width_ff": v[1], "list_width_fw": v[2], "cells": v[3],
                 "outside_grid": v[4], "error_bits": v[5], "sorts": v[6], "reused_lists": v[7] > 0}
 
+    def brick_stats(self) -> dict:
+        """Brick walks of the scene: whether it walks bricks, their shape
+        in cells, the bricks holding particles and the walks whose region
+        exceeded the LDS staging."""
+        v = (c_int64 * 6)()
+        self.ctx.check(self.L.esph_scene_brick_stats(self.handle, v))
+        return {"bricks": bool(v[0]), "brick_cells": [v[1], v[2], v[3]], "occupied": v[4], "spilled": v[5]}
+
     def enable_graph(self, on: bool = True) -> None:
         self.ctx.check(self.L.esph_scene_enable_graph(self.handle, int(on)))
 
