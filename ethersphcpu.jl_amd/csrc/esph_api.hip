@@ -360,8 +360,18 @@ extern "C" int esph_scene_set_buffers(esph_scene* s, const esph_fluid_arrays* in
     return ESPH_OK;
 }
 
-// Cell grid: cells 1.0001 R wide over the bounding box of fluid, walls and
-// finite check bounds (or the caller's domain), one cell of padding.
+// Cell grid over the bounding box of fluid, walls and finite check bounds
+// (or the caller's domain), one cell of padding.  Cells are 1.0001 Rv / 2
+// wide along x (a 5-cell stencil: about 300 candidates per particle in 3D
+// instead of about 500 with Rv-wide cells) and `split` times finer along y
+// and z, split the smallest with cells under 0.8 of the particle spacing
+// there (the first particle's gap; 2 for the examples' h / dx): a row of
+// cells along x then holds at most one lattice line of particles, the
+// particles of a wavefront (consecutive in cell order) lie along one line,
+// and their candidate lists, built row by row, address their neighbours in
+// step -- entry k of consecutive lanes lies on a neighbouring line at
+// consecutive positions, a few cache lines per gather instead of dozens.
+// Cells widen (all axes) while the grid would exceed 2^27 of them.
 static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) {
     const int D = s->d.dim;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -386,30 +396,33 @@ static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) 
     }
     for (int k = 0; k < D; ++k)
         if (!(lo[k] <= hi[k])) { lo[k] = 0.0; hi[k] = 0.0; }
-    // cells half of R wide (a 5-cell stencil per axis: about 300 candidates per
-    // particle in 3D instead of about 500 with R-wide cells), widened when the
-    // grid would exceed 2^27 cells
-    double cs = hd.Rv * 1.0001 / 2.0;
+    double cs0 = hd.Rv * 1.0001 / 2.0;
+    const double spacing = (f->n > 0 && f->gap && std::isfinite(f->gap[0]) && f->gap[0] > 0.0) ? f->gap[0] : cs0;
+    const int split = (int)std::min(4.0, std::max(1.0, std::ceil(cs0 / (0.8 * spacing))));
+    double cs[3];
     for (;;) {
         int64_t ncell = 1;
         for (int k = 0; k < 3; ++k) {
+            cs[k] = k == 0 ? cs0 : cs0 / split;
             if (k >= D) { hd.nc[k] = 1; hd.lo[k] = 0.0; continue; }
-            hd.lo[k] = lo[k] - cs;
-            const int64_t n = (int64_t)std::floor((hi[k] - hd.lo[k]) / cs) + 2;
+            hd.lo[k] = lo[k] - cs[k];
+            const int64_t n = (int64_t)std::floor((hi[k] - hd.lo[k]) / cs[k]) + 2;
             hd.nc[k] = (int32_t)std::min<int64_t>(n, 1 << 20);
             ncell *= hd.nc[k];
         }
         if (ncell <= (int64_t)1 << 27) { hd.ncell = (int32_t)ncell; break; }
-        cs *= 1.25;  // wider cells keep the search exact
+        cs0 *= 1.25;  // wider cells keep the search exact
     }
-    hd.inv_cs = 1.0 / cs;
-    hd.cs = cs;
-    hd.stencil = (int32_t)std::ceil(hd.Rv / cs);
-    if ((double)hd.stencil * cs <= hd.Rv) hd.stencil += 1;
+    double extent = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        hd.cs[k] = cs[k];
+        hd.inv_cs[k] = 1.0 / cs[k];
+        hd.stencil[k] = (int32_t)std::ceil(hd.Rv / cs[k]);
+        if ((double)hd.stencil[k] * cs[k] <= hd.Rv) hd.stencil[k] += 1;
+        if (k < D) extent = std::max(extent, (double)hd.nc[k] * cs[k]);
+    }
     // position quantum (FluidSet::xq): the smallest power of two with the
     // grid extent within 2^30 quanta
-    double extent = 0.0;
-    for (int k = 0; k < D; ++k) extent = std::max(extent, (double)hd.nc[k] * cs);
     const int e = (int)std::ceil(std::log2(std::max(extent, 1e-300) / 1073741824.0));
     hd.qu = std::ldexp(1.0, e);
     hd.inv_qu = std::ldexp(1.0, -e);
@@ -429,7 +442,7 @@ static void build_grid(esph_scene* s, const esph_fluid_arrays* f, SceneDev& hd) 
 
 static int cell_of_host(const SceneDev& hd, int D, const double* x) {
     auto cc = [&](int k) {
-        const double f = std::floor((x[k] - hd.lo[k]) * hd.inv_cs);
+        const double f = std::floor((x[k] - hd.lo[k]) * hd.inv_cs[k]);
         if (!(f >= 0.0)) return 0;
         if (f >= (double)(hd.nc[k] - 1)) return hd.nc[k] - 1;
         return (int)f;
@@ -465,17 +478,16 @@ static bool eos_walk_ok(const esph_scene* s, const esph_fluid_arrays* f) {
 // volumes the longest along x (longer contiguous rows).  False if no brick
 // fits.
 static bool choose_bricks(const SceneDev& hd, double spacing, int bd[3]) {
-    if (!(spacing > 0.0) || !(hd.cs > 0.0)) return false;
-    const int m = hd.stencil;
-    const double a = hd.cs / spacing;
-    auto most = [&](int w) { return std::floor((double)w * a) + 1.0; };
+    if (!(spacing > 0.0) || !(hd.cs[0] > 0.0)) return false;
+    const int mx = hd.stencil[0], my = hd.stencil[1], mz = hd.stencil[2];
+    auto most = [&](int w, int k) { return std::floor((double)w * hd.cs[k] / spacing) + 1.0; };
     int best = 0;
     for (int bx = 1; bx <= 16; ++bx)
         for (int by = 1; by <= 16; ++by)
             for (int bz = 1; bz <= 16; ++bz) {
-                if ((by + 2 * m) * (bz + 2 * m) > kBrickRows || by * bz > kBrickOwn) continue;
-                const double region = most(bx + 2 * m) * most(by + 2 * m) * most(bz + 2 * m);
-                const double own = most(bx) * most(by) * most(bz);
+                if ((by + 2 * my) * (bz + 2 * mz) > kBrickRows || by * bz > kBrickOwn) continue;
+                const double region = most(bx + 2 * mx, 0) * most(by + 2 * my, 1) * most(bz + 2 * mz, 2);
+                const double own = most(bx, 0) * most(by, 1) * most(bz, 2);
                 if (region > 0.92 * kBrickCap || own > kBrickThreads) continue;
                 const int vol = bx * by * bz;
                 if (vol > best || (vol == best && bx > bd[0])) {
@@ -675,8 +687,10 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.key = (int32_t*)dal(I * cap);
     hd.slot = (int32_t*)dal(I * cap);
     hd.tmp = (int32_t*)dal(I * cap);
-    hd.count = (int32_t*)dal(I * (hd.ncell + 1));
-    hd.cell_start = (int32_t*)dal(I * (hd.ncell + 2));
+    // whole scan tiles (the scan reads and writes them in 16-byte words)
+    const size_t ncs = (size_t)hd.cell_tiles * kCellTile + 4;
+    hd.count = (int32_t*)dal(I * ncs);
+    hd.cell_start = (int32_t*)dal(I * ncs);
     hd.bsum = (int32_t*)dal(I * hd.cell_tiles);
     hd.vnbr = (int32_t*)dal(I * (size_t)cap * hd.kff);
     hd.nvnb = (int32_t*)dal(I * (size_t)cap);
@@ -730,7 +744,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hipStream_t st = ctx->stream;
     // the sort expects zero cell counts (its scan clears them after use),
     // and no brick marks
-    HIPCHK(ctx, hipMemsetAsync(hd.count, 0, I * (hd.ncell + 1), st));
+    HIPCHK(ctx, hipMemsetAsync(hd.count, 0, I * ((size_t)hd.cell_tiles * kCellTile + 4), st));
     if (hd.bflag) HIPCHK(ctx, hipMemsetAsync(hd.bflag, 0, brick_flag_bytes(hd.nbricks), st));
     // wall binning and upload
     {

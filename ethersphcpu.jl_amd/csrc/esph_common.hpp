@@ -86,8 +86,9 @@ __device__ __forceinline__ int4 ld_chunk(const int32_t* p) {
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
-// Candidate rows of a point xi in cell c: the grid's cells are at least R /
-// stencil wide, so neighbours lie within `stencil` cells on every axis.  For
+// Candidate rows of a point xi in cell c: the grid's cells are at least Rv /
+// stencil[k] wide along axis k, so neighbours lie within stencil[k] cells
+// along it.  For
 // each (dy, dz) the cells of the row are x-contiguous in memory, so a row is
 // one index range [j0, j1).  Rows and row ends whose nearest point is
 // farther than R from xi (judged from xi's offset inside its cell, with a
@@ -97,53 +98,69 @@ __device__ __forceinline__ int4 ld_chunk(const int32_t* p) {
 template <int D, typename F>
 __device__ __forceinline__ void for_each_row(const SceneDev& s, const int32_t* __restrict__ cstart,
                                              const double* xi, const int c[3], F&& row) {
-    const int m = s.stencil;
-    const double cs = s.cs;
-    const double slack = 1e-7 * cs;
     // trimmed against the candidate radius Rv: between list rebuilds the
     // particles sit up to (Rv - R) / 2 away from the cells they were sorted
-    // into, and Rv covers that
-    const double lim = s.Rv2 * (1.0 + 1e-6);
-    double p[3];
+    // into, and Rv covers that.  In float, with a slack (1e-5 of a cell, and
+    // 1e-5 of Rv^2) far above the float roundings of these few operations on
+    // values below a few cells: the trimming only ever keeps a few more
+    // cells, whose particles the candidate test then rejects.
+    const float lim = (float)(s.Rv2 * (1.0 + 1e-5));
+    float p[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) p[k] = k < D ? xi[k] - (s.lo[k] + (double)c[k] * cs) : 0.0;
-    // distance from xi to the slab of the cell at offset o along one axis
-    auto dmin = [&](int o, double pk) {
-        double d = o == 0 ? 0.0 : (o > 0 ? (double)(o - 1) * cs + (cs - pk) : (double)(-o - 1) * cs + pk);
-        d -= slack;
-        return d > 0.0 ? d : 0.0;
+    for (int k = 0; k < 3; ++k) p[k] = k < D ? (float)(xi[k] - (s.lo[k] + (double)c[k] * s.cs[k])) : 0.0f;
+    // distance from xi to the slab of the cell at offset o along axis k
+    auto dmin = [&](int o, int k) {
+        const float cs = (float)s.cs[k], pk = p[k];
+        float d = o == 0 ? 0.0f : (o > 0 ? (float)(o - 1) * cs + (cs - pk) : (float)(-o - 1) * cs + pk);
+        d -= 1e-5f * cs;
+        return d > 0.0f ? d : 0.0f;
     };
-    const int mz = D == 3 ? m : 0;
+    const int mx = s.stencil[0], my = s.stencil[1], mz = D == 3 ? s.stencil[2] : 0;
+    // the row found last, walked once the next one's range is in flight
+    bool pending = false;
+    int pj0 = 0, pj1 = 0, pcy = 0, pcz = 0;
+    auto emit = [&]() {
+        if constexpr (std::is_invocable_v<F, int, int, int, int>)
+            row(pj0, pj1, pcy, pcz);
+        else
+            row(pj0, pj1);
+    };
     for (int dz = -mz; dz <= mz; ++dz) {
         const int cz = c[2] + dz;
         if (cz < 0 || cz >= s.nc[2]) continue;
-        const double rz = D == 3 ? dmin(dz, p[2]) : 0.0;
-        for (int dy = -m; dy <= m; ++dy) {
+        const float rz = D == 3 ? dmin(dz, 2) : 0.0f;
+        for (int dy = -my; dy <= my; ++dy) {
             const int cy = c[1] + dy;
             if (cy < 0 || cy >= s.nc[1]) continue;
-            const double ry = dmin(dy, p[1]);
-            const double r2 = ry * ry + rz * rz;
+            const float ry = dmin(dy, 1);
+            const float r2 = ry * ry + rz * rz;
             if (r2 > lim) continue;
-            int xl = -m, xh = m;
+            int xl = -mx, xh = mx;
             while (xl < 0) {
-                const double d = dmin(xl, p[0]);
+                const float d = dmin(xl, 0);
                 if (d * d + r2 <= lim) break;
                 ++xl;
             }
             while (xh > 0) {
-                const double d = dmin(xh, p[0]);
+                const float d = dmin(xh, 0);
                 if (d * d + r2 <= lim) break;
                 --xh;
             }
             const int cx0 = c[0] + xl > 0 ? c[0] + xl : 0;
             const int cx1 = c[0] + xh < s.nc[0] - 1 ? c[0] + xh : s.nc[0] - 1;
             if (cx0 > cx1) continue;
-            if constexpr (std::is_invocable_v<F, int, int, int, int>)
-                row(cstart[cell_linear(s, cx0, cy, cz)], cstart[cell_linear(s, cx1, cy, cz) + 1], cy, cz);
-            else
-                row(cstart[cell_linear(s, cx0, cy, cz)], cstart[cell_linear(s, cx1, cy, cz) + 1]);
+            // this row's range is loaded before the previous row is walked,
+            // so the two loads' latency overlaps that walk
+            const int j0 = cstart[cell_linear(s, cx0, cy, cz)], j1 = cstart[cell_linear(s, cx1, cy, cz) + 1];
+            if (pending) emit();
+            pj0 = j0;
+            pj1 = j1;
+            pcy = cy;
+            pcz = cz;
+            pending = true;
         }
     }
+    if (pending) emit();
 }
 
 // Iterates the particles of the candidate rows of xi whose distance
@@ -721,18 +738,19 @@ __device__ __forceinline__ int row_search(const int* a, int n, int v) {
 // Fills the tables of brick b (all threads of the workgroup; ends with a
 // barrier).  Rows outside the grid are empty.
 __device__ __forceinline__ void brick_tables(const SceneDev& s, int b, BrickTables& t) {
-    const int BX = s.bdim[0], BY = s.bdim[1], BZ = s.bdim[2], m = s.stencil;
+    const int BX = s.bdim[0], BY = s.bdim[1], BZ = s.bdim[2];
+    const int mx = s.stencil[0], my = s.stencil[1], mz = s.stencil[2];
     const int bx = b % s.nbk[0], b1 = b / s.nbk[0];
     const int x0 = bx * BX, y0 = (b1 % s.nbk[1]) * BY, z0 = (b1 / s.nbk[1]) * BZ;
-    const int ny = BY + 2 * m, nr = ny * (BZ + 2 * m), no = BY * BZ;
+    const int ny = BY + 2 * my, nr = ny * (BZ + 2 * mz), no = BY * BZ;
     const int32_t* cs = s.cell_start;
     const int tid = (int)threadIdx.x;
     if (tid < nr) {
-        const int cy = y0 - m + tid % ny, cz = z0 - m + tid / ny;
+        const int cy = y0 - my + tid % ny, cz = z0 - mz + tid / ny;
         int st = 0, len = 0;
         if (cy >= 0 && cy < s.nc[1] && cz >= 0 && cz < s.nc[2]) {
-            const int xa = x0 - m > 0 ? x0 - m : 0;
-            const int xb = x0 + BX - 1 + m < s.nc[0] - 1 ? x0 + BX - 1 + m : s.nc[0] - 1;
+            const int xa = x0 - mx > 0 ? x0 - mx : 0;
+            const int xb = x0 + BX - 1 + mx < s.nc[0] - 1 ? x0 + BX - 1 + mx : s.nc[0] - 1;
             st = cs[cell_linear(s, xa, cy, cz)];
             len = cs[cell_linear(s, xb, cy, cz) + 1] - st;
         }
@@ -751,8 +769,8 @@ __device__ __forceinline__ void brick_tables(const SceneDev& s, int b, BrickTabl
         t.ob[r + 1] = len;
     }
     if (tid == 0) {
-        t.y0 = y0 - m;
-        t.z0 = z0 - m;
+        t.y0 = y0 - my;
+        t.z0 = z0 - mz;
         t.ny = ny;
         t.nr = nr;
         t.no = no;

@@ -11,13 +11,13 @@
 namespace esph {
 
 constexpr int kBlock = 256;          // particles per tile == threads per block
-// cells per thread in the scan kernels: 4,096 cells per scan tile.  Each
+// cells per thread in the scan kernels: 16,384 cells per scan tile.  Each
 // tile's block fences its total before the ticket of k_sort_scan1, and that
-// fence writes the L2 back, so fewer, larger tiles make a sort cheaper: 1,024
-// cells per tile 0.586 ms per 3D dam-break step, 4,096 0.580, 8,192 0.579
-// (but the 2D ensemble, with smaller grids, 1.78 ms per batched step
-// against 1.77 at 4,096)
-constexpr int kScanItems = 16;
+// fence writes the L2 back, so fewer, larger tiles make a sort cheaper.
+// Measured with cubic cells (a quarter of today's count): 1,024 cells per
+// tile 0.586 ms per 3D dam-break step, 4,096 0.580, 8,192 0.579 (but the 2D
+// ensemble 1.78 ms per batched step against 1.77 at 4,096)
+constexpr int kScanItems = 64;
 constexpr int kCellTile = kBlock * kScanItems;
 constexpr int kOvfTiles = 16;        // tiles per scene of the overflow (cell-scan) kernels
 // waves per SIMD the force pass of FLAG_EOS_WALK scenes is compiled for
@@ -173,12 +173,14 @@ struct SceneDev {
     double check_planes[8][4];  // deletion iff n . x > d for some plane q < check_nplanes
     int32_t check_nplanes;
     int32_t pad4;
-    // cell grid: cells cs wide (cs >= R / stencil); neighbours of a cell lie
-    // within `stencil` cells on each axis
+    // cell grid: cells cs[k] wide along axis k (cs[k] >= Rv / stencil[k]);
+    // neighbours of a cell lie within stencil[k] cells along axis k.  Cells
+    // are half of Rv along x and finer along y and z (see build_grid), so a
+    // row of cells along x holds about one lattice line of particles
     double lo[3];
-    double inv_cs;
-    double cs;
-    int32_t stencil;
+    double inv_cs[3];
+    double cs[3];
+    int32_t stencil[3];
     int32_t pad1;
     // position quantum of FluidSet::xq: a power of two with the grid extent
     // below 2^30 quanta; qbandq: half-width of the band around R2q = R^2 / qu^2
@@ -283,9 +285,9 @@ __device__ __forceinline__ int cell_coord(double x, double lo, double inv, int n
 
 template <int D>
 __device__ __forceinline__ void cell_of(const SceneDev& s, const double* x, int c[3]) {
-    c[0] = cell_coord(x[0], s.lo[0], s.inv_cs, s.nc[0]);
-    c[1] = cell_coord(x[1], s.lo[1], s.inv_cs, s.nc[1]);
-    c[2] = D == 3 ? cell_coord(x[2], s.lo[2], s.inv_cs, s.nc[2]) : 0;
+    c[0] = cell_coord(x[0], s.lo[0], s.inv_cs[0], s.nc[0]);
+    c[1] = cell_coord(x[1], s.lo[1], s.inv_cs[1], s.nc[1]);
+    c[2] = D == 3 ? cell_coord(x[2], s.lo[2], s.inv_cs[2], s.nc[2]) : 0;
 }
 
 __device__ __forceinline__ int cell_linear(const SceneDev& s, int cx, int cy, int cz) {

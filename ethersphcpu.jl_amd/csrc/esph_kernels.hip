@@ -23,9 +23,9 @@ namespace esph {
 // scan's last phase clears them again once read.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int cell_key(const SceneDev& s, int D, const double* x) {
-    const int cx = cell_coord(x[0], s.lo[0], s.inv_cs, s.nc[0]);
-    const int cy = cell_coord(x[1], s.lo[1], s.inv_cs, s.nc[1]);
-    const int cz = D == 3 ? cell_coord(x[2], s.lo[2], s.inv_cs, s.nc[2]) : 0;
+    const int cx = cell_coord(x[0], s.lo[0], s.inv_cs[0], s.nc[0]);
+    const int cy = cell_coord(x[1], s.lo[1], s.inv_cs[1], s.nc[1]);
+    const int cz = D == 3 ? cell_coord(x[2], s.lo[2], s.inv_cs[2], s.nc[2]) : 0;
     return cell_linear(s, cx, cy, cz);
 }
 
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_bin(const SceneDev* __restrict_
             // counted, since many of them make those cells crowded
             bool far = false;
             for (int k = 0; k < D; ++k) {
-                const double f = floor((x[k] - s.lo[k]) * s.inv_cs);
+                const double f = floor((x[k] - s.lo[k]) * s.inv_cs[k]);
                 far = far || !(f >= -1.0 && f <= (double)s.nc[k]);
             }
             if (far && !dead) atomicAdd(&s.dyn->outside_acc, 1);
@@ -217,20 +217,36 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
         if (!s.dyn->sorted) continue;  // a step without sort
+        // each thread scans kScanItems consecutive cells in 16-byte words
+        // (count and cell_start are allocated in whole tiles)
         const int c0 = tile.begin + (int)threadIdx.x * kScanItems;
         int v[kScanItems];
         int sum = 0;
 #pragma unroll
-        for (int q = 0; q < kScanItems; ++q) {
-            v[q] = c0 + q <= s.ncell ? s.count[c0 + q] : 0;
-            sum += v[q];
+        for (int q = 0; q < kScanItems; q += 4) {
+            const int4 w = *reinterpret_cast<const int4*>(s.count + c0 + q);
+            const int e[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                v[q + k] = c0 + q + k <= s.ncell ? e[k] : 0;
+                sum += v[q + k];
+            }
         }
         int total;
         int run = block_exclusive_scan(sum, &total);
 #pragma unroll
-        for (int q = 0; q < kScanItems; ++q) {
-            if (c0 + q <= s.ncell) s.cell_start[c0 + q] = run;
+        for (int q = 0; q < kScanItems; q += 4) {
+            int4 o;
+            o.x = run;
             run += v[q];
+            o.y = run;
+            run += v[q + 1];
+            o.z = run;
+            run += v[q + 2];
+            o.w = run;
+            run += v[q + 3];
+            // entries past ncell are padding (k_sort_scan3 writes the total at ncell + 1)
+            if (c0 + q <= s.ncell) *reinterpret_cast<int4*>(s.cell_start + c0 + q) = o;
         }
         if (threadIdx.x == 0) {
             s.bsum[tile.begin / kCellTile] = total;
@@ -258,14 +274,30 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan3(const SceneDev* __restric
         const SceneDev& s = scenes[tile.scene];
         if (!s.dyn->sorted) continue;  // a step without sort
         const int off = s.bsum[tile.begin / kCellTile];
+        const int c0 = tile.begin + (int)threadIdx.x * kScanItems;
 #pragma unroll
-        for (int q = 0; q < kScanItems; ++q) {
-            const int c = tile.begin + (int)threadIdx.x * kScanItems + q;
+        for (int q = 0; q < kScanItems; q += 4) {
+            const int c = c0 + q;
             if (c <= s.ncell) {
-                const int st = s.cell_start[c] + off;
-                s.cell_start[c] = st;
-                if (c == s.ncell) s.cell_start[c + 1] = st + s.count[c];
-                s.count[c] = 0;  // ready for the next sort
+                const int4 w = *reinterpret_cast<const int4*>(s.cell_start + c);
+                const int4 n = *reinterpret_cast<const int4*>(s.count + c);
+                const int4 o = make_int4(w.x + off, w.y + off, w.z + off, w.w + off);
+                if (c + 3 <= s.ncell) {
+                    *reinterpret_cast<int4*>(s.cell_start + c) = o;
+                    *reinterpret_cast<int4*>(s.count + c) = make_int4(0, 0, 0, 0);  // ready for the next sort
+                    if (c + 3 == s.ncell) s.cell_start[s.ncell + 1] = o.w + n.w;
+                } else {
+                    // the word holding the last cell: the grand total follows it
+                    const int e[4] = {o.x, o.y, o.z, o.w}, m[4] = {n.x, n.y, n.z, n.w};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (c + k <= s.ncell) {
+                            s.cell_start[c + k] = e[k];
+                            s.count[c + k] = 0;
+                            if (c + k == s.ncell) s.cell_start[c + k + 1] = e[k] + m[k];
+                        }
+                    }
+                }
             }
         }
     }
