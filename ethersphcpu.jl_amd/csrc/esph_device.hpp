@@ -494,6 +494,44 @@ __device__ __forceinline__ double psqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float psqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double prsqrt(double x) { return 1.0 / sqrt(x); }
 __device__ __forceinline__ float prsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+// a / b: IEEE in binary64; through the hardware reciprocal in binary32
+__device__ __forceinline__ double pdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float pdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// r = |r_ij| and ri = 1 / r of a pair from its squared distance d2 >= 0.
+// FAST (binary32, kernels that vanish continuously at R): one hardware
+// reciprocal square root (v_rsq_f32) of max(d2, FLT_MIN) instead of a square
+// root and a reciprocal; ri stays finite and r = 0 at d2 = 0, where every
+// kernel's W' vanishes, so the pair gradient W' ri is 0 there without a test
+// (normalize(0) -> 0, Neighbour.jl:54-59).  The product d2 ri is a few ulp
+// less accurate than the square root, which matters only where W jumps at
+// R: the truncated Gaussian (W(3h) = sigma e^-9) keeps the square root, so a
+// pair the FP64 rule puts just inside R is not evaluated as q = 3.
+// Otherwise (and in binary64: IEEE square root and division) ri = inf at
+// d2 = 0 and the caller zeroes the gradient there.
+template <bool FAST>
+__device__ __forceinline__ void pair_r(float d2, float& r, float& ri) {
+    if (FAST) {
+        ri = __builtin_amdgcn_rsqf(fmaxf(d2, 1.17549435e-38f));
+        r = d2 * ri;
+    } else {
+        r = __builtin_amdgcn_sqrtf(d2);
+        ri = __builtin_amdgcn_rcpf(r);
+    }
+}
+template <bool FAST>
+__device__ __forceinline__ void pair_r(double d2, double& r, double& ri) {
+    r = sqrt(d2);
+    ri = 1.0 / r;
+}
+
+// cos(pi a / b) of the wall force (0 <= a <= b): binary32 by the hardware
+// cosine (v_cos_f32 takes its argument in revolutions, here in [0, 1/2]) and
+// the hardware reciprocal; binary64 by the library, in the reference's order.
+__device__ __forceinline__ float cos_pi_ratio(float a, float b) {
+    return __builtin_amdgcn_cosf(0.5f * a * __builtin_amdgcn_rcpf(b));
+}
+__device__ __forceinline__ double cos_pi_ratio(double a, double b) { return cos(3.14159265358979323846 * a / b); }
 
 // Integer power by squaring (UpdatePressure.jl:13-14 uses x^gamma, gamma::Int).
 __device__ __forceinline__ double ipow(double x, int n) {

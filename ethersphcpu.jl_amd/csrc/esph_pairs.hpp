@@ -28,6 +28,7 @@ template <int D, int K, typename T, bool SCAN, int DEPTH, bool BRICK = false>
 __device__ __forceinline__ void continuity_particle(const SceneDev& s, const SceneDyn& dyn, int i, int dry,
                                                     BrickMap* bm = nullptr, const BrickStage* bst = nullptr) {
     using KQ = KernelQ<T, K>;
+    constexpr bool FASTR = sizeof(T) == 4 && K != ESPH_GAUSSIAN;  // pair_r
     const FluidSet& S = s.set[cur_set(dyn)];
     if (S.kind && S.kind[i] != KIND_FLUID) {
         // inlet/outlet buffers keep rho and p (drho is reset,
@@ -75,11 +76,13 @@ __device__ __forceinline__ void continuity_particle(const SceneDev& s, const Sce
     auto pair = [&](T ih, T in, T d2, const T* rv, const ARec<T>& a) {
         const T* vj = a.v;
         const T m_j = a.m;
-        const T r = psqrt(d2);
+        T r, ri;
+        pair_r<FASTR>(d2, r, ri);
         T W, dW;
         KQ::wdw(r * ih, kA, kB, W, dW);
-        // normalize(0) is NaN and the reference zeroes it (Neighbour.jl:54-59)
-        const T sg = d2 == T(0) ? T(0) : dW * prcp(r);
+        // normalize(0) is NaN and the reference zeroes it (Neighbour.jl:54-59);
+        // with FASTR W'(0) ri is already 0 (pair_r)
+        const T sg = FASTR ? dW * ri : (d2 == T(0) ? T(0) : dW * ri);
         T dot = T(0);
 #pragma unroll
         for (int k = 0; k < D; ++k) dot += (vi[k] - vj[k]) * rv[k];
@@ -264,6 +267,7 @@ template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH, bool EOS
 __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet& S, const FluidSet& N, int i,
                                               bool full_records, const BrickStage* bst = nullptr) {
     using KQ = KernelQ<T, K>;
+    constexpr bool FASTR = sizeof(T) == 4 && K != ESPH_GAUSSIAN;  // pair_r
     if (S.kind && S.kind[i] != KIND_FLUID) {
         buffer_move<D, T>(s, S, N, i);
         return 0.0;
@@ -294,10 +298,12 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     // uniform: the scene's gaps are all equal, so W((gap_i + gap_j) / 2) = W(gap_i)
     // g: the length constants in the units of d2 and rv (QScale)
     auto ff = [&](T in, bool zero, T d2, const T* rv, const Rec<T>& o, auto uniform, const QScale<T>& g) {
-        const T r = psqrt(d2);
+        T r, ri;
+        pair_r<FASTR>(d2, r, ri);
         T W, dW;
         KQ::wdw(r * g.inv_h, kA, kB, W, dW);
-        const T sg = zero ? T(0) : dW * prcp(r);
+        // with FASTR W'(0) ri is already 0 (pair_r)
+        const T sg = FASTR ? dW * ri : (zero ? T(0) : dW * ri);
         // pressure, with the 1% correction W(r) / W((gap_i + gap_j) / 2)
         T P = pr2_i + o.pr2;
         T inv_Wg = inv_Wgap_i;
@@ -309,10 +315,12 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         P += fabs(P) * T(0.01) * W * inv_Wg;
         // viscosity
         const T r2e = r * r + g.eta2;
-        const T vf = T(4) * nu0 * g.len * r * dW * prcp((rho_i + o.rho) * r2e) * in;
-        const T ps = -P * sg * in;
+        const T vf = T(4) * nu0 * g.len * r * dW * prcp((rho_i + o.rho) * r2e);
+        const T ps = -P * sg;
+        // in is 0 or 1, so it weights the pair exactly through the mass
+        const T mw = o.m * in;
 #pragma unroll
-        for (int k = 0; k < D; ++k) dv[k] += o.m * (ps * rv[k] + vf * (me.v[k] - o.v[k]));
+        for (int k = 0; k < D; ++k) dv[k] += mw * (ps * rv[k] + vf * (me.v[k] - o.v[k]));
         if (THERMAL)
             dT += T(4) * g.len * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e) * in;
     };
@@ -330,26 +338,35 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 load_A<D, T>(S.A, j, a.v, a.m, a.gap);
                 return a;
             };
-            auto body = [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t w) {
-                Rec<T> o;
-                o.v[0] = a.v[0];
-                o.v[1] = a.v[1];
-                o.v[2] = a.v[2];
-                o.m = a.m;
-                o.pr2 = eos_pr2(s, __int_as_float(w), o.rho);
-                ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::true_type{}, gq);
+            // the walk in two forms, chosen once per particle: the linear
+            // equation of state (gamma 1, no loop per candidate) or any gamma
+            auto walk = [&](auto lin) {
+                constexpr bool LIN = decltype(lin)::value;
+                auto body = [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t w) {
+                    Rec<T> o;
+                    o.v[0] = a.v[0];
+                    o.v[1] = a.v[1];
+                    o.v[2] = a.v[2];
+                    o.m = a.m;
+                    o.pr2 = eos_pr2<LIN>(s, __int_as_float(w), o.rho);
+                    ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::true_type{}, gq);
+                };
+                if constexpr (BRICK) {
+                    if (bst)
+                        for_pairs_brick<DEPTH>(s, S, xi, i, s.nvnb[i], *bst,
+                                               [&](bool in, float d2, const float* rv, const ARec<float>& a, int32_t w) {
+                            body(0, in, d2, rv, a, w);
+                        });
+                    else  // a region beyond the staging (rare): one gather at a time
+                        for_pairs_q<D, 1, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA, body);
+                } else {
+                    for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA, body);
+                }
             };
-            if constexpr (BRICK) {
-                if (bst)
-                    for_pairs_brick<DEPTH>(s, S, xi, i, s.nvnb[i], *bst,
-                                           [&](bool in, float d2, const float* rv, const ARec<float>& a, int32_t w) {
-                        body(0, in, d2, rv, a, w);
-                    });
-                else  // a region beyond the staging (rare): one gather at a time
-                    for_pairs_q<D, 1, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA, body);
-            } else {
-                for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA, body);
-            }
+            if (s.gamma == 1)
+                walk(std::true_type{});
+            else
+                walk(std::false_type{});
         }
     } else if constexpr (sizeof(T) == 4 && !SCAN && D == 3) {
         // FP32_BULK in 3D: geometry from the quantized positions (FP64 rule
@@ -418,10 +435,12 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 Tw = wa.w;
                 kw = wb.z;
             }
-            const T r = psqrt((T)d2);
+            T r, ri;
+            pair_r<FASTR>((T)d2, r, ri);
             T W, dW;
             KQ::wdw(r * inv_h, kA, kB, W, dW);
-            const T sg = d2 == 0.0 ? T(0) : dW * prcp(r);
+            // with FASTR W'(0) ri is already 0 (pair_r)
+            const T sg = FASTR ? dW * ri : (d2 == 0.0 ? T(0) : dW * ri);
             T vv[3];
             T vn = T(0), rn = T(0);
 #pragma unroll
@@ -435,9 +454,9 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 const T psi = fabs(rn);
                 const T a = r * r - psi * psi;
                 const T xi_ = psqrt(a > T(0) ? a : T(0));
-                const T q = psi / gap_j;
+                const T q = pdiv(psi, gap_j);
                 if (in && !(q > T(1) || xi_ > gap_j)) {
-                    const T p_xi = fabs((T(1) + cos(T(3.14159265358979323846) * xi_ / gap_j)) * T(0.5));
+                    const T p_xi = fabs((T(1) + cos_pi_ratio(xi_, gap_j)) * T(0.5));
                     const T beta = vn > T(0) ? T(0) : T(1);
                     const T wf = (T(0.01) * (c_i * c_i) + beta * c_i * fabs(vn)) * inv_h * fabs(T(1) - q) *
                                  prsqrt(q) * p_xi;
