@@ -716,6 +716,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.wkind = (int32_t*)dal(I * std::max(nw, 1));
     hd.wid = (int32_t*)dal(I * std::max(nw, 1));
     hd.wcell_start = (int32_t*)dal(I * (hd.ncell + 1));
+    hd.wnear = (uint8_t*)dal((size_t)hd.ncell);
     {
         const size_t P = f32 ? F : G;
         hd.wA = dal(4 * P * std::max(nw, 1));
@@ -767,6 +768,28 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         for (int i = 0; i < nw; ++i) wcs[key[i] + 1]++;
         for (int c = 0; c < hd.ncell; ++c) wcs[c + 1] += wcs[c];
         s->wall_stage = WallStage{x, nn, v, gap, T, kap, kind, wid, wcs};
+        // cells with a wall particle within the search stencil (SceneDev::wnear):
+        // the cells holding walls, dilated by stencil[k] along each axis
+        {
+            std::vector<uint8_t>& nr = s->wall_stage.near;
+            nr.assign((size_t)hd.ncell, 0);
+            for (int i = 0; i < nw; ++i) nr[key[i]] = 1;
+            std::vector<int32_t> line;
+            const int64_t stride[3] = {1, hd.nc[0], (int64_t)hd.nc[0] * hd.nc[1]};
+            for (int k = 0; k < D; ++k) {
+                const int n = hd.nc[k], m = hd.stencil[k];
+                line.assign((size_t)n + 1, 0);
+                for (int64_t base = 0; base < hd.ncell; ++base) {
+                    // one line along axis k per base cell whose coordinate k is 0
+                    if ((base / stride[k]) % n != 0) continue;
+                    for (int q = 0; q < n; ++q) line[q + 1] = line[q] + nr[base + q * stride[k]];
+                    for (int q = 0; q < n; ++q) {
+                        const int a = q - m > 0 ? q - m : 0, b = q + m < n - 1 ? q + m : n - 1;
+                        nr[base + q * stride[k]] = line[b + 1] - line[a] > 0 ? 1 : 0;
+                    }
+                }
+            }
+        }
         WallStage& ws = s->wall_stage;
         hipError_t e = hipSuccess;
         auto up = [&](void* dst, const void* src, size_t n) {
@@ -781,6 +804,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         up(hd.wkind, ws.kind.data(), I * nw);
         up(hd.wid, ws.wid.data(), I * nw);
         up(hd.wcell_start, ws.wcs.data(), I * (hd.ncell + 1));
+        up(hd.wnear, ws.near.data(), (size_t)hd.ncell);
         // grid-relative float positions (the neighbour-list prefilter)
         {
             const int W = D == 3 ? 4 : 2;

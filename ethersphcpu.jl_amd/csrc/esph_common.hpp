@@ -314,7 +314,10 @@ __device__ __forceinline__ int rebuild_lists(const SceneDev& s, const FluidSet& 
     cell_of<D>(s, xi, c);
     const int cnt = build_list<D>(s, s.cell_start, S.xf, S.x, xi, c, i, i, s.vnbr, s.kff, bm);
     s.nvnb[i] = cnt;
-    if (s.nw > 0) s.nnbw[i] = build_list<D>(s, s.wcell_start, s.wxf, s.wx, xi, c, i, -1, s.wnbr, s.kfw);
+    if (s.nw > 0)
+        s.nnbw[i] = s.wnear[cell_linear(s, c[0], c[1], c[2])]
+                        ? build_list<D>(s, s.wcell_start, s.wxf, s.wx, xi, c, i, -1, s.wnbr, s.kfw)
+                        : 0;
     return cnt;
 }
 

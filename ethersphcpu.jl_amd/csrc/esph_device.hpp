@@ -222,6 +222,10 @@ struct SceneDev {
     int32_t* wkind;
     int32_t* wid;    // wall index in group-concatenated upload order
     int32_t* wcell_start;  // ncell + 1
+    // per cell: 1 if some wall particle lies in a cell within stencil[k] of
+    // it along every axis k (the cells a candidate search from it visits),
+    // else 0 -- the list rebuild skips the wall search there
+    uint8_t* wnear;
     // packed wall records in the pair precision (static):
     //   3D: wA = (nx, ny, nz, gap)  wB = (vx, vy, vz, T)  wkap = kappa
     //   2D: wA = (nx, ny, gap, T)   wB = (vx, vy, kappa, 0)

@@ -217,36 +217,49 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
         const Tile tile = ctiles[t];
         const SceneDev& s = scenes[tile.scene];
         if (!s.dyn->sorted) continue;  // a step without sort
-        // each thread scans kScanItems consecutive cells in 16-byte words
-        // (count and cell_start are allocated in whole tiles)
-        const int c0 = tile.begin + (int)threadIdx.x * kScanItems;
-        int v[kScanItems];
-        int sum = 0;
+        // each wavefront scans a quarter of the tile, a row of 256 cells
+        // (one 16-byte word per lane, 1 KiB per load) at a time, carrying
+        // the running sum; count and cell_start are allocated in whole tiles
+        constexpr int W = kScanItems / 4;  // words per lane
+        const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+        const int wbase = tile.begin + wave * (kCellTile / (kBlock / 64));
+        int4 v[W];
+        int ex[W];
+        int carry = 0;
 #pragma unroll
-        for (int q = 0; q < kScanItems; q += 4) {
-            const int4 w = *reinterpret_cast<const int4*>(s.count + c0 + q);
-            const int e[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                v[q + k] = c0 + q + k <= s.ncell ? e[k] : 0;
-                sum += v[q + k];
-            }
+        for (int k = 0; k < W; ++k) {
+            const int c = wbase + 256 * k + 4 * lane;
+            int4 w = *reinterpret_cast<const int4*>(s.count + c);
+            // cells past ncell count nothing
+            w.x = c <= s.ncell ? w.x : 0;
+            w.y = c + 1 <= s.ncell ? w.y : 0;
+            w.z = c + 2 <= s.ncell ? w.z : 0;
+            w.w = c + 3 <= s.ncell ? w.w : 0;
+            v[k] = w;
+            const int sum = w.x + w.y + w.z + w.w;
+            const int incl = wave_inclusive_scan(sum);
+            ex[k] = carry + incl - sum;
+            carry += __shfl(incl, 63, 64);
         }
-        int total;
-        int run = block_exclusive_scan(sum, &total);
+        __shared__ int wtot[kBlock / 64];
+        if (lane == 0) wtot[wave] = carry;
+        __syncthreads();
+        int woff = 0, total = 0;
 #pragma unroll
-        for (int q = 0; q < kScanItems; q += 4) {
+        for (int q = 0; q < kBlock / 64; ++q) {
+            woff += q < wave ? wtot[q] : 0;
+            total += wtot[q];
+        }
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int c = wbase + 256 * k + 4 * lane;
             int4 o;
-            o.x = run;
-            run += v[q];
-            o.y = run;
-            run += v[q + 1];
-            o.z = run;
-            run += v[q + 2];
-            o.w = run;
-            run += v[q + 3];
+            o.x = woff + ex[k];
+            o.y = o.x + v[k].x;
+            o.z = o.y + v[k].y;
+            o.w = o.z + v[k].z;
             // entries past ncell are padding (k_sort_scan3 writes the total at ncell + 1)
-            if (c0 + q <= s.ncell) *reinterpret_cast<int4*>(s.cell_start + c0 + q) = o;
+            if (c <= s.ncell) *reinterpret_cast<int4*>(s.cell_start + c) = o;
         }
         if (threadIdx.x == 0) {
             s.bsum[tile.begin / kCellTile] = total;
@@ -274,10 +287,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan3(const SceneDev* __restric
         const SceneDev& s = scenes[tile.scene];
         if (!s.dyn->sorted) continue;  // a step without sort
         const int off = s.bsum[tile.begin / kCellTile];
-        const int c0 = tile.begin + (int)threadIdx.x * kScanItems;
-#pragma unroll
-        for (int q = 0; q < kScanItems; q += 4) {
-            const int c = c0 + q;
+        // 16-byte words across the block (1 KiB per wavefront load)
+#pragma unroll 4
+        for (int q = (int)threadIdx.x; q < kCellTile / 4; q += kBlock) {
+            const int c = tile.begin + 4 * q;
             if (c <= s.ncell) {
                 const int4 w = *reinterpret_cast<const int4*>(s.cell_start + c);
                 const int4 n = *reinterpret_cast<const int4*>(s.count + c);

@@ -64,6 +64,7 @@ struct WallHost {
 struct WallStage {
     std::vector<double> x, n_, v, gap, T, kappa;
     std::vector<int32_t> kind, wid, wcs;
+    std::vector<uint8_t> near;
     std::vector<float> fA, fB, fK, fX;
     std::vector<double> dA, dB, dK;
 };
