@@ -572,6 +572,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     hd.rho0f = (float)d.rho_0;
     hd.inv_rho0f = (float)(1.0 / d.rho_0);
     hd.bf = (float)hd.b;
+    hd.clinf = (float)(hd.b / d.rho_0);
     hd.p0f = (float)d.p_0;
     hd.nu0 = d.mu_0 / d.rho_0;                          // LiquidModel.jl:31
     hd.t0 = d.t_0;

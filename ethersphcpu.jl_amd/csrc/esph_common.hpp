@@ -518,17 +518,20 @@ __device__ __forceinline__ void load_q(const void* xq, int j, int32_t* q) {
 // so that the small difference rho / rho0 - 1 keeps its precision:
 // (rho / rho0)^gamma - 1 = e (1 + u + ... + u^(gamma-1)), e = d / rho0, u = 1 + e;
 // p = b e (...) + p0.  Also returns rho = rho0 + d.
-// LIN: gamma is 1 (the linear equation of state), so the sum is 1 exactly
-// and the loop is left out (the same bits as the loop's).
+// LIN: gamma is 1 and p0 is 0 (the linear equation of state of the
+// examples), so p = (b / rho0) d with no loop.
 template <bool LIN = false>
 __device__ __forceinline__ float eos_pr2(const SceneDev& s, float d, float& rho) {
-    const float e = d * s.inv_rho0f;
-    float g = 1.0f;
-    if (!LIN) {
+    float p;
+    if (LIN) {
+        p = s.clinf * d;
+    } else {
+        const float e = d * s.inv_rho0f;
         const float u = 1.0f + e;
+        float g = 1.0f;
         for (int k = 1; k < s.gamma; ++k) g = g * u + 1.0f;
+        p = s.bf * (e * g) + s.p0f;
     }
-    const float p = LIN ? s.bf * e + s.p0f : s.bf * (e * g) + s.p0f;
     rho = s.rho0f + d;
     const float ir = prcp(rho);
     return p * ir * ir;

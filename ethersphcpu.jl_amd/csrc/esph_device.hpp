@@ -190,6 +190,8 @@ struct SceneDev {
     float qbandq, R2q, pad5, pad6;
     // the equation of state in float for FLAG_EOS_WALK (see eos_pr2)
     float rho0f, inv_rho0f, bf, p0f;
+    float clinf;  // b / rho0: p = clinf (rho - rho0) for gamma 1 and p0 0
+    float pad8;
     // fluid: set[2 * parity + dpar] (see FluidSet)
     FluidSet set[4];
     double* rhon;    // Shepard output / upload staging
@@ -461,31 +463,24 @@ struct KernelQ<T, ESPH_GAUSSIAN> {
     }
 };
 
+// The Wendland polynomials carry a power of t = 2 - q, so t = max(2 - q, 0)
+// gives the zero outside the support without a branch (the same values
+// inside it; -0 for W' outside).
 template <typename T>
 struct KernelQ<T, ESPH_WENDLAND_C2> {
     __device__ static __forceinline__ void wdw(T q, T kA, T kB, T& w, T& dw) {
-        if (q < T(2)) {
-            const T t = T(2) - q, t3 = t * t * t;
-            w = kA * t3 * t * (T(1) + T(2) * q);
-            dw = -kB * q * t3;
-        } else {
-            w = T(0);
-            dw = T(0);
-        }
+        const T t = fmax(T(2) - q, T(0)), t3 = t * t * t;
+        w = kA * t3 * t * (T(1) + T(2) * q);
+        dw = -kB * q * t3;
     }
 };
 
 template <typename T>
 struct KernelQ<T, ESPH_WENDLAND_C4> {
     __device__ static __forceinline__ void wdw(T q, T kA, T kB, T& w, T& dw) {
-        if (q < T(2)) {
-            const T t = T(2) - q, t2 = t * t, t5 = t2 * t2 * t;
-            w = kA * t5 * t * (T(35) * q * q + T(36) * q + T(12));
-            dw = -kB * (T(35.0 / 96.0) * q * q + T(7.0 / 48.0) * q) * t5;
-        } else {
-            w = T(0);
-            dw = T(0);
-        }
+        const T t = fmax(T(2) - q, T(0)), t2 = t * t, t5 = t2 * t2 * t;
+        w = kA * t5 * t * (T(35) * q * q + T(36) * q + T(12));
+        dw = -kB * (T(35.0 / 96.0) * q * q + T(7.0 / 48.0) * q) * t5;
     }
 };
 
@@ -504,7 +499,7 @@ __device__ __forceinline__ float pdiv(float a, float b) { return a * __builtin_a
 
 // r = |r_ij| and ri = 1 / r of a pair from its squared distance d2 >= 0.
 // FAST (binary32, kernels that vanish continuously at R): one hardware
-// reciprocal square root (v_rsq_f32) of max(d2, FLT_MIN) instead of a square
+// reciprocal square root (v_rsq_f32) of d2 + FLT_MIN instead of a square
 // root and a reciprocal; ri stays finite and r = 0 at d2 = 0, where every
 // kernel's W' vanishes, so the pair gradient W' ri is 0 there without a test
 // (normalize(0) -> 0, Neighbour.jl:54-59).  The product d2 ri is a few ulp
@@ -516,7 +511,8 @@ __device__ __forceinline__ float pdiv(float a, float b) { return a * __builtin_a
 template <bool FAST>
 __device__ __forceinline__ void pair_r(float d2, float& r, float& ri) {
     if (FAST) {
-        ri = __builtin_amdgcn_rsqf(fmaxf(d2, 1.17549435e-38f));
+        // d2 + FLT_MIN equals d2 for every d2 a pair can have but 0
+        ri = __builtin_amdgcn_rsqf(d2 + 1.17549435e-38f);
         r = d2 * ri;
     } else {
         r = __builtin_amdgcn_sqrtf(d2);

@@ -284,6 +284,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     T W_gap, dW_gap;
     KQ::wdw(gap_i * inv_h, kA, kB, W_gap, dW_gap);
     const T inv_Wgap_i = prcp(W_gap);
+    const T c01_i = T(0.01) * inv_Wgap_i;
     T dv[3] = {T(0), T(0), T(0)};
     T dT = T(0);
     // candidates beyond R contribute exactly zero (weighted, not branched)
@@ -306,21 +307,36 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         const T sg = FASTR ? dW * ri : (zero ? T(0) : dW * ri);
         // pressure, with the 1% correction W(r) / W((gap_i + gap_j) / 2)
         T P = pr2_i + o.pr2;
-        T inv_Wg = inv_Wgap_i;
-        if (!decltype(uniform)::value && o.gap != gap_i) {
-            T w2, d2w;
-            KQ::wdw((gap_i + o.gap) * T(0.5) * inv_h, kA, kB, w2, d2w);
-            inv_Wg = prcp(w2);
+        if (sizeof(T) == 4 && decltype(uniform)::value) {
+            // one gap: 0.01 / W(gap_i) is the particle's constant
+            P += fabs(P) * W * c01_i;
+        } else {
+            T inv_Wg = inv_Wgap_i;
+            if (!decltype(uniform)::value && o.gap != gap_i) {
+                T w2, d2w;
+                KQ::wdw((gap_i + o.gap) * T(0.5) * inv_h, kA, kB, w2, d2w);
+                inv_Wg = prcp(w2);
+            }
+            P += fabs(P) * T(0.01) * W * inv_Wg;
         }
-        P += fabs(P) * T(0.01) * W * inv_Wg;
         // viscosity
         const T r2e = r * r + g.eta2;
         const T vf = T(4) * nu0 * g.len * r * dW * prcp((rho_i + o.rho) * r2e);
         const T ps = -P * sg;
         // in is 0 or 1, so it weights the pair exactly through the mass
         const T mw = o.m * in;
+        if (sizeof(T) == 4) {
+            // the two terms as two multiply-adds per component
+            const T a = mw * ps, b = mw * vf;
 #pragma unroll
-        for (int k = 0; k < D; ++k) dv[k] += mw * (ps * rv[k] + vf * (me.v[k] - o.v[k]));
+            for (int k = 0; k < D; ++k) {
+                dv[k] += a * rv[k];
+                dv[k] += b * (me.v[k] - o.v[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; ++k) dv[k] += mw * (ps * rv[k] + vf * (me.v[k] - o.v[k]));
+        }
         if (THERMAL)
             dT += T(4) * g.len * kap_i * o.kap * r * dW * (T_i - o.Tt) * o.m * prcp((kap_i + o.kap) * o.rho * r2e) * in;
     };
@@ -332,14 +348,16 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
             // p / rho^2 from the equation of state, for the candidates and,
             // alike, for this particle
             float rho_e;
-            pr2_i = eos_pr2(s, reinterpret_cast<const float*>(S.xq)[4 * (size_t)i + 3], rho_e);
+            const bool lin = s.gamma == 1 && s.p0f == 0.0f;
+            const float d_i = reinterpret_cast<const float*>(S.xq)[4 * (size_t)i + 3];
+            pr2_i = lin ? eos_pr2<true>(s, d_i, rho_e) : eos_pr2<false>(s, d_i, rho_e);
             auto loadA = [&](int j) {
                 ARec<T> a;
                 load_A<D, T>(S.A, j, a.v, a.m, a.gap);
                 return a;
             };
             // the walk in two forms, chosen once per particle: the linear
-            // equation of state (gamma 1, no loop per candidate) or any gamma
+            // equation of state (gamma 1, p0 0: no loop per candidate) or any
             auto walk = [&](auto lin) {
                 constexpr bool LIN = decltype(lin)::value;
                 auto body = [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t w) {
@@ -363,7 +381,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                     for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA, body);
                 }
             };
-            if (s.gamma == 1)
+            if (lin)
                 walk(std::true_type{});
             else
                 walk(std::false_type{});
