@@ -1059,11 +1059,12 @@ __device__ __forceinline__ bool shepard_due(const SceneDev& s, const SceneDyn& d
 
 static inline int grid_for(int ntiles) { return ntiles > 0 ? ntiles : 1; }
 
+// The pair passes' launch over the bucket's particle tiles (overflowed
+// particles scan their cells inside the same launch).
 template <typename L>
 static hipError_t launch_both(const Bucket& b, hipStream_t st, L&& launch) {
     if (b.ntiles <= 0) return hipSuccess;
     launch(std::false_type{}, b.tiles, b.ntiles);
-    if (!b.tuning) launch(std::true_type{}, b.ovf_tiles, kOvfTiles * b.nscenes);
     return hipGetLastError();
 }
 

@@ -56,9 +56,10 @@ __device__ __forceinline__ void continuity_particle(const SceneDev& s, const Sce
         }
         if (dry && overflowed(s, i)) return;
         if (overflowed(s, i)) {
-            // finished by the SCAN instance of this kernel
-            s.ovf[atomicAdd(&s.dyn->novf, 1)] = i;
+            // more candidates than the list holds (rare): this thread scans
+            // the particle's cells instead (same pair set)
             atomicAdd(&s.dyn->overflow, 1);
+            continuity_particle<D, K, T, true, 1>(s, dyn, i, dry);
             return;
         }
     }
@@ -597,10 +598,11 @@ __attribute__((amdgpu_waves_per_eu(EOSW && !SCAN ? kEosWalkWaves
         if (!SCAN && (s.flags & FLAG_BRICK)) continue;  // k_forces_brick
         double dmax = 0.0;
         for_work<SCAN>(s, dyn, tile, first, [&](int i) {
-            if (!SCAN && overflowed(s, i)) return;
-            const double d2 =
-                forces_body<D, K, T, THERMAL, SCAN, DEPTH, EOSW>(s, s.set[cur_set(dyn)], s.set[next_set(dyn)], i,
-                                                                 dyn.rebuild != 0);
+            const FluidSet &S = s.set[cur_set(dyn)], &N = s.set[next_set(dyn)];
+            // a particle whose candidates overflowed its list (rare) scans its cells
+            const double d2 = !SCAN && overflowed(s, i)
+                                  ? forces_body<D, K, T, THERMAL, true, 1, EOSW>(s, S, N, i, dyn.rebuild != 0)
+                                  : forces_body<D, K, T, THERMAL, SCAN, DEPTH, EOSW>(s, S, N, i, dyn.rebuild != 0);
             dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
         });
         if (s.verlet) record_disp(s, dmax);
@@ -619,9 +621,11 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(4
         const BrickStage st{lq, la};
         double dmax = 0.0;
         for_brick_particles(tb, [&](int i) {
-            if (overflowed(s, i)) return;
-            const double d2 = forces_body<3, K, float, false, false, DEPTH, true, true>(
-                s, s.set[cur_set(dyn)], s.set[next_set(dyn)], i, dyn.rebuild != 0, spill ? nullptr : &st);
+            const FluidSet &S = s.set[cur_set(dyn)], &N = s.set[next_set(dyn)];
+            const double d2 = overflowed(s, i)
+                                  ? forces_body<3, K, float, false, true, 1, true>(s, S, N, i, dyn.rebuild != 0)
+                                  : forces_body<3, K, float, false, false, DEPTH, true, true>(
+                                        s, S, N, i, dyn.rebuild != 0, spill ? nullptr : &st);
             dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
         });
         if (s.verlet) record_disp(s, dmax);
@@ -633,11 +637,53 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(4
 // ReconstructValue.jl:22-57): Shepard filter over the step-start pairs with
 // the post-continuity density, in binary64 in every form.
 // ---------------------------------------------------------------------------
+// One particle of the Shepard pass (k_shepard).
+template <int D, int K, bool SCAN, int DEPTH>
+__device__ __forceinline__ void shepard_particle(const SceneDev& s, const SceneDyn& dyn, int i) {
+    using KQ = KernelQ<double, K>;
+    const FluidSet& S = s.set[cur_set(dyn)];
+    const int8_t* kind = S.kind;
+    if (kind && kind[i] != KIND_FLUID) return;
+    double xi[3];
+    load_x<D>(S.x, i, xi);
+    double sw = 0.0, ss = 0.0;
+    struct SRec {
+        double m, rho;
+        int8_t kind;
+    };
+    auto load = [&](int j) {
+        SRec r;
+        r.m = S.m[j];
+        r.rho = S.rho[j];
+        r.kind = kind ? kind[j] : (int8_t)KIND_FLUID;
+        return r;
+    };
+    const double R2 = s.R2;
+    for_pairs_d<D, SCAN, DEPTH, true, D == 3>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
+                                [&](int, double d2, const double*, const SRec& o) {
+        // weight 1 for neighbours under the exact rule among fluid particles
+        const double in = d2 <= R2 && o.kind == KIND_FLUID ? 1.0 : 0.0;
+        double W, dW;
+        KQ::wdw(sqrt(d2) * s.inv_h, s.kA, s.kB, W, dW);
+        const double rho_j = o.rho;
+        const double a = W * o.m / rho_j * in;
+        sw += a;
+        ss += a * rho_j;
+    });
+    {
+#pragma clang fp contract(off)
+        const double rho_i = S.rho[i];
+        const double a = s.W0 * S.m[i] / rho_i;
+        sw += a;
+        ss += a * rho_i;
+        s.rhon[i] = ss / sw;
+    }
+}
+
 // force (tuner runs): whether or not a re-initialisation is due.
 template <int D, int K, bool SCAN, int DEPTH, int BS>
 __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* __restrict__ scenes,
                                                     const Tile* __restrict__ tiles, int ntiles, int force) {
-    using KQ = KernelQ<double, K>;
     constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
     const int nv = ntiles * SUB;
     for (int vb = blockIdx.x; vb < nv; vb += gridDim.x) {
@@ -651,44 +697,11 @@ __global__ __launch_bounds__(SCAN ? kBlock : BS) void k_shepard(const SceneDev* 
         // tuner run or a skipped pass leaves the densities as they are)
         if (!force && threadIdx.x == 0) s.dyn->shepard_ran = 1;
         for_work<SCAN>(s, dyn, tile, first, [&](int i) {
-        if (!SCAN && overflowed(s, i)) return;
-        const FluidSet& S = s.set[cur_set(dyn)];
-        const int8_t* kind = S.kind;
-        if (kind && kind[i] != KIND_FLUID) return;
-        double xi[3];
-        load_x<D>(S.x, i, xi);
-        double sw = 0.0, ss = 0.0;
-        struct SRec {
-            double m, rho;
-            int8_t kind;
-        };
-        auto load = [&](int j) {
-            SRec r;
-            r.m = S.m[j];
-            r.rho = S.rho[j];
-            r.kind = kind ? kind[j] : (int8_t)KIND_FLUID;
-            return r;
-        };
-        const double R2 = s.R2;
-        for_pairs_d<D, SCAN, DEPTH, true, D == 3>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i], load,
-                                    [&](int, double d2, const double*, const SRec& o) {
-            // weight 1 for neighbours under the exact rule among fluid particles
-            const double in = d2 <= R2 && o.kind == KIND_FLUID ? 1.0 : 0.0;
-            double W, dW;
-            KQ::wdw(sqrt(d2) * s.inv_h, s.kA, s.kB, W, dW);
-            const double rho_j = o.rho;
-            const double a = W * o.m / rho_j * in;
-            sw += a;
-            ss += a * rho_j;
-        });
-        {
-#pragma clang fp contract(off)
-            const double rho_i = S.rho[i];
-            const double a = s.W0 * S.m[i] / rho_i;
-            sw += a;
-            ss += a * rho_i;
-            s.rhon[i] = ss / sw;
-        }
+            // a particle whose candidates overflowed its list (rare) scans its cells
+            if (!SCAN && overflowed(s, i))
+                shepard_particle<D, K, true, 1>(s, dyn, i);
+            else
+                shepard_particle<D, K, SCAN, DEPTH>(s, dyn, i);
         });
     }
 }
@@ -739,8 +752,6 @@ hipError_t launch_continuity_d(const Bucket& b, hipStream_t st) {
                 });
             }
         }
-        // the overflowed particles, listed by the passes above
-        if (!b.tuning) launch(std::true_type{}, b.ovf_tiles, kOvfTiles * b.nscenes);
         e = hipGetLastError();
     });
     return e;
@@ -769,8 +780,7 @@ hipError_t launch_forces_t(const Bucket& b, hipStream_t st, bool main = true) {
 
 // One force-pass instance per kind of scene in the bucket: thermal, plain,
 // (3D FP32_BULK) the two-record walk of FLAG_EOS_WALK scenes, and their brick
-// walk (FLAG_BRICK; the two-record instance then runs for their overflowed
-// particles only, unless other scenes need it).
+// walk (FLAG_BRICK).
 template <int D>
 hipError_t launch_forces_d(const Bucket& b, hipStream_t st) {
     hipError_t e = hipSuccess;
@@ -778,7 +788,7 @@ hipError_t launch_forces_d(const Bucket& b, hipStream_t st) {
     if (e == hipSuccess && b.has_plain) e = launch_forces_t<D, false>(b, st);
     if constexpr (D == 3) {
         if (b.prec == ESPH_FP32_BULK) {
-            if (e == hipSuccess && (b.has_eos || b.has_brick)) e = launch_forces_t<D, false, true>(b, st, b.has_eos);
+            if (e == hipSuccess && b.has_eos) e = launch_forces_t<D, false, true>(b, st);
             if (e == hipSuccess && b.has_brick && b.ntiles > 0) {
                 ESPH_DISPATCH_K(b.kernel, {
                     ESPH_DISPATCH_DEPTH(b.depth, {
