@@ -165,19 +165,20 @@ __device__ __forceinline__ void for_each_row(const SceneDev& s, const int32_t* _
 
 // Iterates the particles of the candidate rows of xi whose distance
 // satisfies the neighbour rule (FP64, exact).  The candidate loop issues the
-// loads of four candidates before testing them.
-template <int D, typename F>
+// loads of U candidates before testing them (U = 1 where the scan is the
+// rare fallback inside a pass, to keep that pass's registers).
+template <int D, int U = 4, typename F>
 __device__ __forceinline__ void for_each_neighbour(const SceneDev& s, const int32_t* __restrict__ cstart,
                                                    const double* __restrict__ xall, const double* xi,
                                                    const int c[3], int self, F&& f) {
     const double R2 = s.R2;
     for_each_row<D>(s, cstart, xi, c, [&](int j0, int j1) {
-        for (int j = j0; j < j1; j += 4) {
-            double xj[4][3];
+        for (int j = j0; j < j1; j += U) {
+            double xj[U][3];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) load_x<D>(xall, j + u < j1 ? j + u : j1 - 1, xj[u]);
+            for (int u = 0; u < U; ++u) load_x<D>(xall, j + u < j1 ? j + u : j1 - 1, xj[u]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const int jj = j + u;
                 double rv[3];
                 const double d2 = dist2_exact<D>(xi, xj[u], rv);
@@ -405,9 +406,10 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
                                             const int32_t* __restrict__ list, int cnt, Load&& load, Body&& body) {
     static_assert(DEPTH == 1 || DEPTH == 2 || DEPTH == 4, "DEPTH divides a 4-entry chunk");
     if (SCAN) {
+        // the overflow fallback inside a pass: one candidate at a time
         int c[3];
         cell_of<D>(s, xi, c);
-        for_each_neighbour<D>(s, cstart, xall, xi, c, self, [&](int j, double d2, const double* rv) {
+        for_each_neighbour<D, 1>(s, cstart, xall, xi, c, self, [&](int j, double d2, const double* rv) {
             auto r = load(j);
             if constexpr (has_late<std::decay_t<Load>>::value) load.late(j, r);
             body(j, d2, rv, r);
