@@ -692,6 +692,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     const size_t ncs = (size_t)hd.cell_tiles * kCellTile + 4;
     hd.count = (int32_t*)dal(I * ncs);
     hd.cell_start = (int32_t*)dal(I * ncs);
+    hd.cell_local = (int32_t*)dal(I * ncs);
     hd.bsum = (int32_t*)dal(I * hd.cell_tiles);
     hd.vnbr = (int32_t*)dal(I * (size_t)cap * hd.kff);
     hd.nvnb = (int32_t*)dal(I * (size_t)cap);

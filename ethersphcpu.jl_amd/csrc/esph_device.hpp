@@ -199,6 +199,7 @@ struct SceneDev {
     int32_t* slot;
     int32_t* count;       // ncell + 1
     int32_t* cell_start;  // ncell + 2
+    int32_t* cell_local;  // prefix of the counts within their scan tile (the running sort)
     int32_t* tmp;         // cap
     int32_t* bsum;        // scan block sums
     int32_t cell_tiles;   // ceil((ncell+1) / kCellTile)

@@ -205,9 +205,9 @@ __device__ __forceinline__ void compact_bricks(const SceneDev& s) {
     if (threadIdx.x == 0) s.dyn->nbrick = total;
 }
 
-// Phases 1 and 2: scan of kCellTile counts per tile, tile total to bsum; the
-// last of a scene's tiles to finish (a ticket counter, no waiting) then scans
-// the scene's tile totals.  Release: each block's total is fenced before its
+// Phases 1 and 2: scan of kCellTile counts per tile into cell_local, tile
+// total to bsum; the last of a scene's tiles to finish (a ticket counter, no
+// waiting) then scans the scene's tile totals.  Release: each block's total is fenced before its
 // ticket; acquire: the last block fences after its ticket, before reading.
 __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ ctiles, int ntiles) {
@@ -258,8 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
             o.y = o.x + v[k].x;
             o.z = o.y + v[k].y;
             o.w = o.z + v[k].z;
-            // entries past ncell are padding (k_sort_scan3 writes the total at ncell + 1)
-            if (c <= s.ncell) *reinterpret_cast<int4*>(s.cell_start + c) = o;
+            // tile-local prefixes; k_sort_offsets_place adds the tile offsets
+            if (c <= s.ncell) *reinterpret_cast<int4*>(s.cell_local + c) = o;
         }
         if (threadIdx.x == 0) {
             s.bsum[tile.begin / kCellTile] = total;
@@ -277,64 +277,70 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
     }
 }
 
-// Phase 3: add tile offsets; the last cell also writes the grand total.
-// The counts are read for the last time here and cleared.
-__global__ __launch_bounds__(kBlock) void k_sort_scan3(const SceneDev* __restrict__ scenes,
-                                                       const Tile* __restrict__ ctiles, int ntiles) {
-    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
-        const int t = tile_index(tb, ntiles);
-        const Tile tile = ctiles[t];
-        const SceneDev& s = scenes[tile.scene];
-        if (!s.dyn->sorted) continue;  // a step without sort
-        const int off = s.bsum[tile.begin / kCellTile];
-        // 16-byte words across the block (1 KiB per wavefront load)
+// Phase 3 and placement in one launch: blocks [0, nct) turn the cell tiles'
+// local prefixes (cell_local, from k_sort_scan1) into cell_start by adding
+// their tile offsets, write the grand total after the last cell and clear
+// the counts for the next sort; blocks [nct, nct + npt) place each particle
+// of a particle tile at cell start + slot in tmp, with the cell start taken
+// from cell_local and the tile offsets directly (neither part reads what the
+// other writes).
+__device__ __forceinline__ int cell_start_of(const SceneDev& s, int c) {
+    return s.cell_local[c] + s.bsum[c / kCellTile];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_offsets_place(const SceneDev* __restrict__ scenes,
+                                                               const Tile* __restrict__ ctiles, int nct,
+                                                               const Tile* __restrict__ ptiles, int npt,
+                                                               int mode) {
+    for (int tb = blockIdx.x; tb < nct + npt; tb += gridDim.x) {
+        if (tb < nct) {
+            const Tile tile = ctiles[tb];
+            const SceneDev& s = scenes[tile.scene];
+            if (!s.dyn->sorted) continue;  // a step without sort
+            const int off = s.bsum[tile.begin / kCellTile];
+            // 16-byte words across the block (1 KiB per wavefront load)
 #pragma unroll 4
-        for (int q = (int)threadIdx.x; q < kCellTile / 4; q += kBlock) {
-            const int c = tile.begin + 4 * q;
-            if (c <= s.ncell) {
-                const int4 w = *reinterpret_cast<const int4*>(s.cell_start + c);
-                const int4 n = *reinterpret_cast<const int4*>(s.count + c);
-                const int4 o = make_int4(w.x + off, w.y + off, w.z + off, w.w + off);
-                if (c + 3 <= s.ncell) {
-                    *reinterpret_cast<int4*>(s.cell_start + c) = o;
-                    *reinterpret_cast<int4*>(s.count + c) = make_int4(0, 0, 0, 0);  // ready for the next sort
-                    if (c + 3 == s.ncell) s.cell_start[s.ncell + 1] = o.w + n.w;
-                } else {
-                    // the word holding the last cell: the grand total follows it
-                    const int e[4] = {o.x, o.y, o.z, o.w}, m[4] = {n.x, n.y, n.z, n.w};
+            for (int q = (int)threadIdx.x; q < kCellTile / 4; q += kBlock) {
+                const int c = tile.begin + 4 * q;
+                if (c <= s.ncell) {
+                    const int4 w = *reinterpret_cast<const int4*>(s.cell_local + c);
+                    const int4 n = *reinterpret_cast<const int4*>(s.count + c);
+                    const int4 o = make_int4(w.x + off, w.y + off, w.z + off, w.w + off);
+                    if (c + 3 <= s.ncell) {
+                        *reinterpret_cast<int4*>(s.cell_start + c) = o;
+                        *reinterpret_cast<int4*>(s.count + c) = make_int4(0, 0, 0, 0);  // ready for the next sort
+                        if (c + 3 == s.ncell) s.cell_start[s.ncell + 1] = o.w + n.w;
+                    } else {
+                        // the word holding the last cell: the grand total follows it
+                        const int ev[4] = {o.x, o.y, o.z, o.w}, m[4] = {n.x, n.y, n.z, n.w};
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (c + k <= s.ncell) {
-                            s.cell_start[c + k] = e[k];
-                            s.count[c + k] = 0;
-                            if (c + k == s.ncell) s.cell_start[c + k + 1] = e[k] + m[k];
+                        for (int k = 0; k < 4; ++k) {
+                            if (c + k <= s.ncell) {
+                                s.cell_start[c + k] = ev[k];
+                                s.count[c + k] = 0;
+                                if (c + k == s.ncell) s.cell_start[c + k + 1] = ev[k] + m[k];
+                            }
                         }
                     }
                 }
             }
-        }
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_sort_place(const SceneDev* __restrict__ scenes,
-                                                       const Tile* __restrict__ tiles, int ntiles, int mode) {
-    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
-        const int t = tile_index(tb, ntiles);
-        const Tile tile = tiles[t];
-        const SceneDev& s = scenes[tile.scene];
-        if (!s.dyn->sorted) continue;  // a step without sort
-        const int i = tile.begin + (int)threadIdx.x;
-        const SceneDyn dyn = *s.dyn;
-        if (i < dyn.n_live) {
-            s.tmp[s.cell_start[s.key[i]] + s.slot[i]] = i;
-        } else if (mode == 0 && (s.flags & FLAG_IO) && i < dyn.n_live + spawn_count(s, dyn)) {
-            // a fluid copy of an entering inlet particle; its rank among this
-            // step's copies follows the inlet order (push! in that order)
-            const int e = i - dyn.n_live, ns = spawn_count(s, dyn), my = s.spawn_id[e];
-            int r = 0;
-            for (int q = 0; q < ns; ++q) r += s.spawn_id[q] < my ? 1 : 0;
-            s.spawn_order[r] = e;
-            s.tmp[s.cell_start[s.spawn_key[e]] + s.spawn_slot[e]] = dyn.n_live + r;
+        } else {
+            const Tile tile = ptiles[tb - nct];
+            const SceneDev& s = scenes[tile.scene];
+            if (!s.dyn->sorted) continue;  // a step without sort
+            const int i = tile.begin + (int)threadIdx.x;
+            const SceneDyn dyn = *s.dyn;
+            if (i < dyn.n_live) {
+                s.tmp[cell_start_of(s, s.key[i]) + s.slot[i]] = i;
+            } else if (mode == 0 && (s.flags & FLAG_IO) && i < dyn.n_live + spawn_count(s, dyn)) {
+                // a fluid copy of an entering inlet particle; its rank among this
+                // step's copies follows the inlet order (push! in that order)
+                const int ei = i - dyn.n_live, ns = spawn_count(s, dyn), my = s.spawn_id[ei];
+                int r = 0;
+                for (int q = 0; q < ns; ++q) r += s.spawn_id[q] < my ? 1 : 0;
+                s.spawn_order[r] = ei;
+                s.tmp[cell_start_of(s, s.spawn_key[ei]) + s.spawn_slot[ei]] = dyn.n_live + r;
+            }
         }
     }
 }
@@ -714,8 +720,8 @@ hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st) {
     dim3 blk(kBlock);
     hipLaunchKernelGGL(k_sort_bin, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_scan1, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
-    hipLaunchKernelGGL(k_sort_scan3, dim3(grid_for(p.nctiles)), blk, 0, st, p.scenes, p.ctiles, p.nctiles);
-    hipLaunchKernelGGL(k_sort_place, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
+    hipLaunchKernelGGL(k_sort_offsets_place, dim3(grid_for(p.nctiles + p.nptiles)), blk, 0, st, p.scenes, p.ctiles,
+                       p.nctiles, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_scatter, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_finish, dim3((p.nscenes + 63) / 64), dim3(64), 0, st, p.scenes, p.nscenes, mode);
     return hipGetLastError();
