@@ -146,6 +146,12 @@ extern "C" int esph_ctx_set_brick_walk(esph_ctx* ctx, int32_t enable) {
     return ESPH_OK;
 }
 
+extern "C" int esph_ctx_set_wall_lanes(esph_ctx* ctx, int32_t enable) {
+    if (!ctx) return ESPH_ERR_ARG;
+    ctx->wall_lanes = enable != 0;
+    return ESPH_OK;
+}
+
 extern "C" int esph_ctx_tuned_workgroup(const esph_ctx* ctx, int32_t dim, int32_t kernel, int32_t precision,
                                         int32_t thermal_mix, int32_t* threads, float* ms3) {
     if (!ctx || !threads) return ESPH_ERR_ARG;
@@ -525,7 +531,8 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     // (FluidWallSolve.jl:68-79, FluidInletOutletWallSolve.jl:121-122)
     const bool wall_p = d.solver != ESPH_SOLVER_FLUID_WALL && !io;
     hd.flags = (thermal ? FLAG_THERMAL : 0) | (wall_p ? FLAG_WALL_PRESSURE : 0) | (d.check_enabled && !io ? FLAG_CHECK : 0) |
-               (io ? FLAG_IO : 0) | (eos_walk_ok(s, f) ? FLAG_EOS_WALK : 0);
+               (io ? FLAG_IO : 0) | (eos_walk_ok(s, f) ? FLAG_EOS_WALK : 0) |
+               (ctx->wall_lanes ? FLAG_WALL_LANES : 0);
     hd.io_axis = d.io_axis;
     hd.inlet_enter = d.inlet_enter;
     hd.inlet_shift = d.inlet_shift;

@@ -694,6 +694,60 @@ __device__ __forceinline__ int wave_inclusive_scan(int x) {
     return x;
 }
 
+// DPP moves (GFX9 controls): by n lanes toward higher lanes inside each
+// 16-lane row (kDppRowShr + n), lane 15 of each row to the next row, lane 31
+// to rows 2 and 3; lanes outside the row mask, or whose source lies outside
+// their row, get 0.  Every lane of the wavefront must be active.
+constexpr int kDppRowShr = 0x110, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Segmented inclusive sum over the 64 lanes of a wavefront, every lane
+// active: lane p gets the sum of lanes head..p, head <= p being the first
+// lane of p's segment (segments are runs of consecutive lanes).  The steps
+// of a Hillis-Steele scan as DPP moves -- shifts by 1, 2, 4 and 8 inside the
+// rows, then the broadcasts of lanes 15 and 31 -- each adding the moved value
+// only where it comes from the same segment.  The order of the additions
+// depends only on the segment's lanes, and so does the result.
+template <typename T>
+__device__ __forceinline__ T wave_segmented_sum(T v, int head) {
+    const int lane = (int)(threadIdx.x & 63);
+    T y;
+    y = dpp_move<kDppRowShr + 1, 0xf>(v);
+    v += lane - 1 >= head ? y : T(0);
+    y = dpp_move<kDppRowShr + 2, 0xf>(v);
+    v += lane - 2 >= head ? y : T(0);
+    y = dpp_move<kDppRowShr + 4, 0xf>(v);
+    v += lane - 4 >= head ? y : T(0);
+    y = dpp_move<kDppRowShr + 8, 0xf>(v);
+    v += lane - 8 >= head ? y : T(0);
+    y = dpp_move<kDppRowBcast15, 0xa>(v);  // rows 1 and 3: lanes 15 and 47
+    v += head < (lane & ~15) ? y : T(0);
+    y = dpp_move<kDppRowBcast31, 0xc>(v);  // rows 2 and 3: lane 31
+    v += lane >= 32 && head < 32 ? y : T(0);
+    return v;
+}
+
+// Orders LDS accesses of one wavefront's lanes to each other (its LDS
+// instructions complete in program order; this keeps the compiler's order).
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // Brick walk (FLAG_BRICK).  Bricks are blocks of bdim cells; a workgroup
 // takes one brick at a time, stages the quantized positions and A records of

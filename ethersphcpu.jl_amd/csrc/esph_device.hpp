@@ -50,6 +50,10 @@ enum : int {
     // cells with their neighbour regions staged in LDS (SceneDev::bdim);
     // candidates are addressed by their LDS slots (SceneDev::vslot)
     FLAG_BRICK = 32,
+    // the force pass walks each particle's wall candidates in its own lane
+    // (and brick walks always do); without it the candidates of a wavefront's
+    // particles are shared out over all its lanes (wall_walk_shared)
+    FLAG_WALL_LANES = 64,
 };
 
 // Particle kinds of the inlet/outlet solver (FluidSet::kind).

@@ -255,6 +255,228 @@ __device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S
     if (N.T) N.T[i] = S.T[i];
 }
 
+// ---------------------------------------------------------------------------
+// Wall terms of one fluid particle and one wall candidate: wallForce!
+// (WallForce.jl:8-30, Rogers & Dalrymple 2008), pressureForce! wall
+// (PressureForce.jl:23-35) and viscosityForce! wall (ViscosityForce.jl:27-46),
+// added to acc; r, W'(r) and r^2 + 0.01 h^2 are returned for the thermostat
+// term.  in: the exact rule d^2 <= R^2 (candidates beyond it add nothing).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct WallConst {
+    T inv_h, kA, kB, eta2, nu0;
+    T P0;           // p0 / rho0^2, the wall's p / rho^2
+    bool pressure;  // FLAG_WALL_PRESSURE
+    double R2;
+};
+
+// Windows of 64 wall candidates that wall_walk_shared gathers at a time
+// (two: four take more than the 128 registers of the float force pass).
+constexpr int kWallWindows = 2;
+// Wavefronts with wall candidates in more lanes than this walk them lane by
+// lane (per-wavefront choice of the force pass's wall walk).
+constexpr int kWallShareLanes = 16;
+
+// The particle's side of the wall terms.
+template <typename T>
+struct WallOwner {
+    double x[3];
+    T v[3];
+    T c, pr2, inv_rho, m, inv_wg;  // sound speed, p / rho^2, 1 / rho, mass, 1 / W(gap)
+    int32_t ex, i;                 // first position in the wavefront's sequence (wall_walk_shared), particle
+};
+
+// Wall records (SceneDev::wA, wB) of candidate j.
+template <typename T>
+__device__ __forceinline__ void load_wall(const SceneDev& s, int j, typename Vec4<T>::type& wa,
+                                          typename Vec4<T>::type& wb) {
+    using V = typename Vec4<T>::type;
+    if constexpr (sizeof(V) == 16) {
+        wa = ldrec<V>(s.wA, j);
+        wb = ldrec<V>(s.wB, j);
+    } else {
+        wa = static_cast<const V*>(s.wA)[j];
+        wb = static_cast<const V*>(s.wB)[j];
+    }
+}
+
+// Normal, velocity, gap, temperature and conductivity of a wall record pair
+// (kwv: the 3D conductivity, kept apart).
+template <int D, typename T, typename V>
+__device__ __forceinline__ void unpack_wall(const V& wa, const V& wb, T kwv, T* n, T* vw, T& gap_j, T& Tw, T& kw) {
+    n[0] = wa.x;
+    n[1] = wa.y;
+    vw[0] = wb.x;
+    vw[1] = wb.y;
+    if (D == 3) {
+        n[2] = wa.z;
+        gap_j = wa.w;
+        vw[2] = wb.z;
+        Tw = wb.w;
+        kw = kwv;
+    } else {
+        n[2] = T(0);
+        vw[2] = T(0);
+        gap_j = wa.z;
+        Tw = wa.w;
+        kw = wb.z;
+    }
+}
+
+template <int D, int K, typename T, bool FASTR>
+__device__ __forceinline__ void wall_pair(const WallConst<T>& w, const WallOwner<T>& o, bool in, double d2,
+                                          const double* rv, const T* n, const T* vw, T gap_j, T* acc, T& r, T& dW,
+                                          T& r2e) {
+    using KQ = KernelQ<T, K>;
+    T ri;
+    pair_r<FASTR>((T)d2, r, ri);
+    T W;
+    KQ::wdw(r * w.inv_h, w.kA, w.kB, W, dW);
+    // with FASTR W'(0) ri is already 0 (pair_r)
+    const T sg = FASTR ? dW * ri : (d2 == 0.0 ? T(0) : dW * ri);
+    T vv[3];
+    T vn = T(0), rn = T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        vv[k] = o.v[k] - vw[k];
+        vn += vv[k] * n[k];
+        rn += (T)rv[k] * n[k];
+    }
+    // wallForce! (Rogers & Dalrymple 2008)
+    {
+        const T psi = fabs(rn);
+        const T a = r * r - psi * psi;
+        const T xi_ = psqrt(a > T(0) ? a : T(0));
+        const T q = pdiv(psi, gap_j);
+        if (in && !(q > T(1) || xi_ > gap_j)) {
+            const T p_xi = fabs((T(1) + cos_pi_ratio(xi_, gap_j)) * T(0.5));
+            const T beta = vn > T(0) ? T(0) : T(1);
+            const T wf =
+                (T(0.01) * (o.c * o.c) + beta * o.c * fabs(vn)) * w.inv_h * fabs(T(1) - q) * prsqrt(q) * p_xi;
+#pragma unroll
+            for (int k = 0; k < D; ++k) acc[k] += wf * n[k];
+        }
+    }
+    r2e = r * r + w.eta2;
+    T vf = T(2) * w.nu0 * r * dW * o.inv_rho * prcp(r2e);
+    T ps = T(0);
+    if (w.pressure) {
+        T P = o.pr2 + w.P0;
+        P += fabs(P) * T(0.01) * W * o.inv_wg;
+        ps = -P * sg;
+    }
+    vf = in ? vf : T(0);
+    ps = in ? ps : T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc[k] += o.m * (ps * (T)rv[k] + vf * (vv[k] - vn * n[k]));
+}
+
+// The wall walk of a wavefront's particles with their candidates shared out
+// over all its lanes (called by every lane together; cnt: the lane's wall
+// candidates, 0 for lanes without a particle to advance).  A few particles
+// of a wavefront lie near a wall -- along a lattice line, the first three
+// next to an end wall -- and walking their lists in their own lanes (20 to
+// 60 candidates each) leaves the other lanes idle: on the 1M-particle 3D dam
+// break a tenth of the wall walk's lane slots did work.  Here the lists of
+// lanes 0..63, in lane and list order, form one sequence of positions, cut
+// into windows of 64: in window b lane p takes position b + p, finds whose
+// it is (the last particle starting at or before it; own[] marks the starts
+// inside the window) and that particle's inputs (slot[], the wavefront's LDS
+// copy of every lane's WallOwner), adds up the candidate's terms, and a
+// segmented sum over the window gives each particle the total of its
+// positions there, read from its last one.  kWallWindows windows at a time:
+// their list entries, then their wall records, are gathered together, so a
+// group costs two memory round trips (the walk's time is those waits: the
+// pass is latency-bound, and a wavefront near a wall waits for its walls
+// after its fluid walk).  slot and own: the workgroup's areas, 64 and
+// 64 kWallWindows entries per wavefront.
+template <int D, int K, typename T, bool FASTR>
+__device__ __forceinline__ void wall_walk_shared(const SceneDev& s, const WallConst<T>& wc, int cnt,
+                                                 const WallOwner<T>& me, WallOwner<T>* slot, int* own, T* dv) {
+    using V = typename Vec4<T>::type;
+    constexpr int G = kWallWindows;
+    const int lane = (int)(threadIdx.x & 63);
+    // this wavefront's entries (the offset is uniform: a scalar register)
+    const unsigned w0 = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
+    slot += w0;
+    own += w0 * G;
+    const int incl = wave_segmented_sum(cnt, 0);  // inclusive prefix sum over the lanes
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    if (total == 0) return;
+    const int ex = incl - cnt;
+    {
+        WallOwner<T> o = me;
+        o.ex = ex;
+        slot[lane] = o;
+    }
+    const size_t cap = (size_t)s.cap;
+    int carry = 0;  // whose the last position of the previous window was
+    for (int b0 = 0; b0 < total; b0 += 64 * G) {
+        // the starts of the group's windows: own[64 u + p] for position b0 + 64 u + p
+        wave_lds_fence();
+#pragma unroll
+        for (int u = 0; u < G; ++u) own[64 * u + lane] = -1;
+        const int rel = ex - b0;
+        if (cnt > 0 && rel >= 0 && rel < 64 * G) own[rel] = lane;
+        wave_lds_fence();
+        int head[G], o[G], j[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const uint64_t starts = __ballot(own[64 * u + lane] >= 0);
+            const uint64_t upto = starts & (lane == 63 ? ~0ull : (2ull << lane) - 1ull);
+            // the window lane where this position's particle starts (0: before the window)
+            head[u] = upto ? 63 - __clzll((long long)upto) : 0;
+            const int m = own[64 * u + head[u]];
+            o[u] = upto ? m : carry;
+            carry = __builtin_amdgcn_readlane(o[u], 63);
+        }
+        // list entries, then wall records, of every window of the group
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int g = b0 + 64 * u + lane;
+            j[u] = -1;
+            if (g < total) {
+                const WallOwner<T>& ow = slot[o[u]];
+                j[u] = s.wnbr[list_at(cap, g - ow.ex, ow.i)];
+            }
+        }
+        double xj[G][3];
+        V wa[G], wb[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (j[u] >= 0) {
+                load_x<D>(s.wx, j[u], xj[u]);
+                load_wall<T>(s, j[u], wa[u], wb[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int b = b0 + 64 * u;
+            if (b >= total) break;
+            T acc[3] = {T(0), T(0), T(0)};
+            if (j[u] >= 0) {
+                const WallOwner<T> ow = slot[o[u]];
+                double rv[3];
+                const double d2 = dist2_exact<D>(ow.x, xj[u], rv);
+                T n[3], vw[3], gap_j, Tw, kw;
+                unpack_wall<D, T>(wa[u], wb[u], T(0), n, vw, gap_j, Tw, kw);
+                T r, dW, r2e;
+                wall_pair<D, K, T, FASTR>(wc, ow, d2 <= wc.R2, d2, rv, n, vw, gap_j, acc, r, dW, r2e);
+            }
+#pragma unroll
+            for (int k = 0; k < D; ++k) acc[k] = wave_segmented_sum(acc[k], head[u]);
+            // this lane's particle: the sum at its last position in the window
+            const bool mine = cnt > 0 && ex < b + 64 && ex + cnt > b;
+            const int last = mine ? min(ex + cnt - b, 64) - 1 : lane;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const T t = __shfl(acc[k], last, 64);
+                if (mine) dv[k] += t;
+            }
+        }
+    }
+}
+
 // Returns the particle's squared displacement from its position when the
 // candidate lists were built (0 when the lists are rebuilt every step).
 // full_records: the next copy's records were last written in another
@@ -264,14 +486,34 @@ __device__ __forceinline__ void buffer_move(const SceneDev& s, const FluidSet& S
 // BRICK (FLAG_BRICK scenes): the fluid walk reads the candidates from the
 // staged brick region bst (null when the region exceeded kBrickCap: from
 // device memory, as the other scenes).
-template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH, bool EOSW, bool BRICK = false>
+// WAVE: called by every lane of the wavefront together (act: the lane has a
+// particle i to advance; the others take part in the shared wall walk
+// only), with the workgroup's LDS areas for wall_walk_shared.
+template <int D, int K, typename T, bool THERMAL, bool SCAN, int DEPTH, bool EOSW, bool BRICK = false,
+          bool WAVE = false>
 __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet& S, const FluidSet& N, int i,
-                                              bool full_records, const BrickStage* bst = nullptr) {
+                                              bool full_records, const BrickStage* bst = nullptr, bool act = true,
+                                              WallOwner<T>* wslot = nullptr, int* wown = nullptr) {
     using KQ = KernelQ<T, K>;
     constexpr bool FASTR = sizeof(T) == 4 && K != ESPH_GAUSSIAN;  // pair_r
-    if (S.kind && S.kind[i] != KIND_FLUID) {
+    static_assert(!WAVE || (!THERMAL && !SCAN && !BRICK), "wavefront form: plain list walks");
+    bool live = act;
+    if (live && S.kind && S.kind[i] != KIND_FLUID) {
         buffer_move<D, T>(s, S, N, i);
-        return 0.0;
+        live = false;
+    }
+    if constexpr (!WAVE) {
+        if (!live) return 0.0;
+    } else if (!live) {
+        i = 0;  // in range (capacity >= 1); nothing of it is walked or written
+    }
+    // candidates of the fluid walk
+    const int nv = live ? s.nvnb[i] : 0;
+    // (wavefront form) wall candidates, loaded ahead: the wall walk after the
+    // fluid walk starts from them
+    int cw = 0;
+    if constexpr (WAVE) {
+        if (s.nw > 0 && live) cw = s.nnbw[i];
     }
     const T inv_h = (T)s.inv_h, kA = (T)s.kA, kB = (T)s.kB;
     const T eta2 = (T)(0.01 * (s.h * s.h));
@@ -372,14 +614,14 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 };
                 if constexpr (BRICK) {
                     if (bst)
-                        for_pairs_brick<DEPTH>(s, S, xi, i, s.nvnb[i], *bst,
+                        for_pairs_brick<DEPTH>(s, S, xi, i, nv, *bst,
                                                [&](bool in, float d2, const float* rv, const ARec<float>& a, int32_t w) {
                             body(0, in, d2, rv, a, w);
                         });
                     else  // a region beyond the staging (rare): one gather at a time
-                        for_pairs_q<D, 1, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA, body);
+                        for_pairs_q<D, 1, false>(s, S, xi, i, s.vnbr, nv, loadA, body);
                 } else {
-                    for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], loadA, body);
+                    for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, nv, loadA, body);
                 }
             };
             if (lin)
@@ -391,12 +633,12 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
         // FP32_BULK in 3D: geometry from the quantized positions (FP64 rule
         // near R).  The 2D ensemble measured 4% slower with them (1.90
         // against 1.83 ms per batched step) and keeps the binary64 ones.
-        for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, s.nvnb[i], RecLoaderAB<D, T, THERMAL>{S},
+        for_pairs_q<D, DEPTH, false>(s, S, xi, i, s.vnbr, nv, RecLoaderAB<D, T, THERMAL>{S},
                                      [&](int, bool in, float d2, const float* rv, const Rec<T>& o, int32_t) {
             ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::false_type{}, gq);
         });
     } else {
-        for_pairs_d<D, SCAN, DEPTH, false>(s, s.cell_start, S.x, xi, i, i, s.vnbr, s.nvnb[i],
+        for_pairs_d<D, SCAN, DEPTH, false>(s, s.cell_start, S.x, xi, i, i, s.vnbr, nv,
                                            RecLoaderAB<D, T, THERMAL>{S},
                                            [&](int, double d2, const double* rv, const Rec<T>& o) {
             const T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
@@ -406,99 +648,69 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
     if (THERMAL) dT *= prcp(rho_i * (T)S.cp[i]);
     if (s.nw > 0) {
         using V = typename Vec4<T>::type;
-        const bool wall_pressure = (s.flags & FLAG_WALL_PRESSURE) != 0;
-        const T c_i = (T)S.c[i];
-        const T P0 = (T)(s.p0 / (s.rho0 * s.rho0));
-        const T inv_rho_i = prcp(rho_i);
-        const T th_w = (T)(1.0 / (s.cp * s.rho0)) * inv_rho_i * m_i;
-        T dTw = T(0);
-        struct WRec {
-            V wa, wb;
-            T kw;
-        };
-        auto wload = [&](int j) {
-            WRec w;
-            if constexpr (sizeof(V) == 16) {
-                w.wa = ldrec<V>(s.wA, j);
-                w.wb = ldrec<V>(s.wB, j);
-            } else {
-                w.wa = static_cast<const V*>(s.wA)[j];
-                w.wb = static_cast<const V*>(s.wB)[j];
-            }
-            w.kw = (D == 3 && THERMAL) ? ld<T>(s.wkap, j) : T(0);
-            return w;
-        };
-        const double R2 = s.R2;
-        // (brick walks: at most two wall gathers in flight, which keeps the
-        // four-deep fluid walk within the registers of four waves per SIMD)
-        constexpr int WDEPTH = BRICK && DEPTH > 2 ? 2 : DEPTH;
-        for_pairs_d<D, SCAN, WDEPTH, SPLIT>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, s.nnbw[i], wload,
-                                    [&](int j, double d2, const double* rv, const WRec& w) {
-            // candidates within Rv: the exact rule selects the terms (no branch)
-            const bool in = d2 <= R2;
-            const V wa = w.wa;
-            const V wb = w.wb;
-            T n[3], vw[3], gap_j, Tw, kw;
-            n[0] = wa.x;
-            n[1] = wa.y;
-            vw[0] = wb.x;
-            vw[1] = wb.y;
-            if (D == 3) {
-                n[2] = wa.z;
-                gap_j = wa.w;
-                vw[2] = wb.z;
-                Tw = wb.w;
-                kw = w.kw;
-            } else {
-                gap_j = wa.z;
-                Tw = wa.w;
-                kw = wb.z;
-            }
-            T r, ri;
-            pair_r<FASTR>((T)d2, r, ri);
-            T W, dW;
-            KQ::wdw(r * inv_h, kA, kB, W, dW);
-            // with FASTR W'(0) ri is already 0 (pair_r)
-            const T sg = FASTR ? dW * ri : (d2 == 0.0 ? T(0) : dW * ri);
-            T vv[3];
-            T vn = T(0), rn = T(0);
+        WallConst<T> wc;
+        wc.inv_h = inv_h;
+        wc.kA = kA;
+        wc.kB = kB;
+        wc.eta2 = eta2;
+        wc.nu0 = nu0;
+        wc.P0 = (T)(s.p0 / (s.rho0 * s.rho0));
+        wc.pressure = (s.flags & FLAG_WALL_PRESSURE) != 0;
+        wc.R2 = s.R2;
+        WallOwner<T> own;
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                vv[k] = me.v[k] - vw[k];
-                vn += vv[k] * n[k];
-                rn += (T)rv[k] * n[k];
-            }
-            // wallForce! (Rogers & Dalrymple 2008)
-            {
-                const T psi = fabs(rn);
-                const T a = r * r - psi * psi;
-                const T xi_ = psqrt(a > T(0) ? a : T(0));
-                const T q = pdiv(psi, gap_j);
-                if (in && !(q > T(1) || xi_ > gap_j)) {
-                    const T p_xi = fabs((T(1) + cos_pi_ratio(xi_, gap_j)) * T(0.5));
-                    const T beta = vn > T(0) ? T(0) : T(1);
-                    const T wf = (T(0.01) * (c_i * c_i) + beta * c_i * fabs(vn)) * inv_h * fabs(T(1) - q) *
-                                 prsqrt(q) * p_xi;
-#pragma unroll
-                    for (int k = 0; k < D; ++k) dv[k] += wf * n[k];
-                }
-            }
-            const T r2e = r * r + eta2;
-            T vf = T(2) * nu0 * r * dW * inv_rho_i * prcp(r2e);
-            T ps = T(0);
-            if (wall_pressure) {
-                T P = pr2_i + P0;
-                P += fabs(P) * T(0.01) * W * inv_Wgap_i;
-                ps = -P * sg;
-            }
-            vf = in ? vf : T(0);
-            ps = in ? ps : T(0);
-#pragma unroll
-            for (int k = 0; k < D; ++k) dv[k] += m_i * (ps * (T)rv[k] + vf * (vv[k] - vn * n[k]));
-            if (THERMAL && in && s.wkind[j] == ESPH_WALL_THERMOSTATIC)
-                dTw += (kap_i + kw) * r * dW * (T_i - Tw) * prcp(r2e);
-        });
-        if (THERMAL) dT += dTw * th_w;
+        for (int k = 0; k < 3; ++k) {
+            own.x[k] = xi[k];
+            own.v[k] = me.v[k];
+        }
+        own.c = (T)S.c[i];
+        own.pr2 = pr2_i;
+        own.inv_rho = prcp(rho_i);
+        own.m = m_i;
+        own.inv_wg = inv_Wgap_i;
+        own.ex = 0;
+        own.i = i;
+        bool lanes = true;
+        if constexpr (WAVE) {
+            // shared out when few lanes have wall candidates (next to an end
+            // wall); a wavefront along a floor or side wall, with candidates
+            // in most lanes, walks them lane by lane, its gathers four deep
+            lanes = (s.flags & FLAG_WALL_LANES) != 0 || __popcll(__ballot(cw > 0)) > kWallShareLanes;
+        }
+        if (!lanes) {
+            if constexpr (WAVE) wall_walk_shared<D, K, T, FASTR>(s, wc, cw, own, wslot, wown, dv);
+        } else if (live) {
+            const T th_w = (T)(1.0 / (s.cp * s.rho0)) * own.inv_rho * m_i;
+            T dTw = T(0);
+            struct WRec {
+                V wa, wb;
+                T kw;
+            };
+            auto wload = [&](int j) {
+                WRec w;
+                load_wall<T>(s, j, w.wa, w.wb);
+                w.kw = (D == 3 && THERMAL) ? ld<T>(s.wkap, j) : T(0);
+                return w;
+            };
+            // (brick walks: at most two wall gathers in flight, which keeps the
+            // four-deep fluid walk within the registers of four waves per SIMD)
+            constexpr int WDEPTH = BRICK && DEPTH > 2 ? 2 : DEPTH;
+            for_pairs_d<D, SCAN, WDEPTH, SPLIT>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, WAVE ? cw : s.nnbw[i], wload,
+                                                [&](int j, double d2, const double* rv, const WRec& w) {
+                // candidates within Rv: the exact rule selects the terms (no branch)
+                const bool in = d2 <= wc.R2;
+                T n[3], vw[3], gap_j, Tw, kw;
+                unpack_wall<D, T>(w.wa, w.wb, w.kw, n, vw, gap_j, Tw, kw);
+                T r, dW, r2e;
+                wall_pair<D, K, T, FASTR>(wc, own, in, d2, rv, n, vw, gap_j, dv, r, dW, r2e);
+                if (THERMAL && in && s.wkind[j] == ESPH_WALL_THERMOSTATIC)
+                    dTw += (kap_i + kw) * r * dW * (T_i - Tw) * prcp(r2e);
+            });
+            if (THERMAL) dT += dTw * th_w;
+        }
+    }
+    if constexpr (WAVE) {
+        if (!live) return 0.0;
     }
     {
 #pragma clang fp contract(off)
@@ -586,6 +798,11 @@ __attribute__((amdgpu_waves_per_eu(EOSW && !SCAN ? kEosWalkWaves
                                    : std::is_same<T, float>::value && !THERMAL && !SCAN ? 4 : 1))) void k_forces(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, int ntiles) {
     constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
+    // the wavefront form of the body (list walks without heat): LDS for the
+    // shared wall walk, 64 entries per wavefront
+    constexpr bool WAVE = !SCAN && !THERMAL;
+    __shared__ WallOwner<T> wslot[WAVE ? BS : 1];
+    __shared__ int wown[WAVE ? BS * kWallWindows : 1];
     const int nv = ntiles * SUB;
     for (int vb = blockIdx.x; vb < nv; vb += gridDim.x) {
         const int v = tile_index(vb, nv);
@@ -597,14 +814,29 @@ __attribute__((amdgpu_waves_per_eu(EOSW && !SCAN ? kEosWalkWaves
         if (((s.flags & FLAG_EOS_WALK) != 0) != EOSW) continue;
         if (!SCAN && (s.flags & FLAG_BRICK)) continue;  // k_forces_brick
         double dmax = 0.0;
-        for_work<SCAN>(s, dyn, tile, first, [&](int i) {
+        if constexpr (WAVE) {
+            // every lane of the wavefront runs the body together (the wall
+            // walk shares candidates out over the lanes); a particle whose
+            // candidates overflowed its list (rare) then scans its cells on
+            // its own
+            const int i = first + (int)threadIdx.x;
+            const bool act = i < dyn.n_live;
             const FluidSet &S = s.set[cur_set(dyn)], &N = s.set[next_set(dyn)];
-            // a particle whose candidates overflowed its list (rare) scans its cells
-            const double d2 = !SCAN && overflowed(s, i)
-                                  ? forces_body<D, K, T, THERMAL, true, 1, EOSW>(s, S, N, i, dyn.rebuild != 0)
-                                  : forces_body<D, K, T, THERMAL, SCAN, DEPTH, EOSW>(s, S, N, i, dyn.rebuild != 0);
-            dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
-        });
+            const bool ovf = act && overflowed(s, i);
+            dmax = forces_body<D, K, T, THERMAL, false, DEPTH, EOSW, false, true>(
+                s, S, N, i, dyn.rebuild != 0, nullptr, act && !ovf, wslot, wown);
+            if (ovf) dmax = forces_body<D, K, T, THERMAL, true, 1, EOSW>(s, S, N, i, dyn.rebuild != 0);
+        } else {
+            for_work<SCAN>(s, dyn, tile, first, [&](int i) {
+                const FluidSet &S = s.set[cur_set(dyn)], &N = s.set[next_set(dyn)];
+                // a particle whose candidates overflowed its list (rare) scans its cells
+                const double d2 = !SCAN && overflowed(s, i)
+                                      ? forces_body<D, K, T, THERMAL, true, 1, EOSW>(s, S, N, i, dyn.rebuild != 0)
+                                      : forces_body<D, K, T, THERMAL, SCAN, DEPTH, EOSW>(s, S, N, i,
+                                                                                       dyn.rebuild != 0);
+                dmax = d2 > dmax || d2 != d2 ? d2 : dmax;  // NaN propagates
+            });
+        }
         if (s.verlet) record_disp(s, dmax);
     }
 }

@@ -34,6 +34,9 @@ struct esph_ctx {
     // 1M-particle 3D dam break the force pass takes 0.38 ms with them against
     // 0.27 ms gathering from device memory
     int bricks = 0;
+    // wall walks of the force pass in each particle's own lane (FLAG_WALL_LANES)
+    // instead of shared out over the wavefront, decided at upload; off by default
+    int wall_lanes = 0;
     std::map<std::array<int, 4>, int> depth;
     std::map<std::array<int, 4>, std::array<float, 3>> depth_ms;  // forces pass at depth 1, 2, 4 (256 threads)
     std::map<std::array<int, 4>, int> block;

@@ -97,6 +97,7 @@ def lib() -> ctypes.CDLL:
         "esph_ctx_set_tuning": ([vp, c_int32, c_int32], c_int32),
         "esph_ctx_set_workgroup": ([vp, c_int32], c_int32),
         "esph_ctx_set_brick_walk": ([vp, c_int32], c_int32),
+        "esph_ctx_set_wall_lanes": ([vp, c_int32], c_int32),
         "esph_ctx_tuned_workgroup": ([vp, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_float)],
                                      c_int32),
         "esph_ctx_tuned_depth": ([vp, c_int32, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_float)],
@@ -196,6 +197,12 @@ class Context:
         scenes that allow them (default off); applies to scenes uploaded
         afterwards.  Results are identical either way."""
         self.check(self._L.esph_ctx_set_brick_walk(self.handle, 1 if enable else 0))
+
+    def set_wall_lanes(self, enable: bool = True) -> None:
+        """Force-pass wall walks in each particle's own lane (True) or shared
+        out over the wavefront's lanes (default); applies to scenes uploaded
+        afterwards.  The two differ in the rounding of the wall sums only."""
+        self.check(self._L.esph_ctx_set_wall_lanes(self.handle, 1 if enable else 0))
 
     def tuned_workgroup(self, dim: int, kernel: int, precision: int, thermal_mix: int = 1):
         """(threads, [ms at 64, 128, 256 threads]) of a tuned class, or None."""

@@ -23,10 +23,15 @@ def say(*a):
     sys.stdout.flush()
 
 
+# Both contexts walk each particle's wall candidates in its own lane, as
+# brick walks always do (the shared wall walk sums a wavefront's wall terms
+# in an order that depends on the wavefront's particles, which differ
+# between the two walks).
 @pytest.fixture(scope="module")
 def brick_ctx():
     c = _lib.Context(0)
     c.set_brick_walk(True)
+    c.set_wall_lanes(True)
     yield c
     c.close()
 
@@ -35,6 +40,7 @@ def brick_ctx():
 def gather_ctx():
     c = _lib.Context(0)
     c.set_brick_walk(False)
+    c.set_wall_lanes(True)
     yield c
     c.close()
 
