@@ -643,7 +643,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     size_t bytes = 2 * per_set + (size_t)cap * (G + 3 * I) +
                    (size_t)(hd.ncell + 1) * I * 2 + I + (size_t)hd.cell_tiles * I +
                    (size_t)nw * (3 * D * G + 3 * G + 2 * I + 9 * P) + (size_t)(hd.ncell + 1) * I +
-                   sizeof(SceneDyn) + sizeof(SceneDev) +
+                   2 * sizeof(SceneDyn) + sizeof(SceneDev) +
                    ((size_t)(cap + kBlock - 1) / kBlock + (size_t)hd.cell_tiles) * sizeof(Tile) +
                    (size_t)cap * I * (hd.kff + hd.kfw + 4) + 2 * (size_t)cap * 4 * F + (f32 ? 2 * (size_t)cap * 4 * I : 0) +
                    (size_t)cap * D * G + brick_bytes;
@@ -739,7 +739,8 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
         hd.vslot = (uint16_t*)dal(sizeof(uint16_t) * (size_t)cap * hd.kff);
         if (eos_walk_ok(s, f)) hd.flags |= FLAG_BRICK;
     }
-    hd.dyn = (SceneDyn*)dal(sizeof(SceneDyn));
+    hd.dyn = (SceneDyn*)dal(2 * sizeof(SceneDyn));
+    hd.snap = hd.dyn + 1;
     s->dev = (SceneDev*)dal(sizeof(SceneDev));
     s->nptiles = (cap + kBlock - 1) / kBlock;
     s->nctiles = hd.cell_tiles;

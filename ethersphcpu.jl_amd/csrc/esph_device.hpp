@@ -80,7 +80,7 @@ struct SceneDyn {
     int32_t next_fluid_id;   // next sequence number of the fluid group
     int32_t next_outlet_id;  // next sequence number of the outlet group
     int32_t outside;   // particles more than one cell outside the grid at the last sort
-    int32_t rebuild;   // 1: the next continuity pass rebuilds the candidate lists (a sort just ran)
+    int32_t rebuild;   // 1: a sort just ran (new particle order: lists rebuilt, records written whole)
     int32_t sorted;    // this step's decision: sort (and rebuild) or commit in place
     uint32_t disp2_bits;  // float bits of this step's largest squared displacement from xref
     int32_t nsorts;       // sorts (list rebuilds) so far, the upload's included
@@ -263,6 +263,10 @@ struct SceneDev {
     int32_t* spawn_order; // spawn index by rank (rank = order of the inlet sequence numbers)
     int32_t* leave_id;    // fluid sequence numbers of this step's fluid -> outlet moves
     SceneDyn* dyn;
+    // a copy of *dyn taken by the sort pass before its scatter (by
+    // k_sort_offsets_place): the scatter reads it while the scene's first
+    // tile ends the step in *dyn (sort_finish)
+    SceneDyn* snap;
 };
 
 template <typename T>

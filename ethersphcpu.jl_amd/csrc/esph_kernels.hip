@@ -283,7 +283,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
 // the counts for the next sort; blocks [nct, nct + npt) place each particle
 // of a particle tile at cell start + slot in tmp, with the cell start taken
 // from cell_local and the tile offsets directly (neither part reads what the
-// other writes).
+// other writes).  The block of each scene's first cell tile also copies the
+// scene's step state to SceneDev::snap for k_sort_scatter, sort or not.
 __device__ __forceinline__ int cell_start_of(const SceneDev& s, int c) {
     return s.cell_local[c] + s.bsum[c / kCellTile];
 }
@@ -296,6 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_offsets_place(const SceneDev* _
         if (tb < nct) {
             const Tile tile = ctiles[tb];
             const SceneDev& s = scenes[tile.scene];
+            if (tile.begin == 0 && threadIdx.x == 0) *s.snap = *s.dyn;
             if (!s.dyn->sorted) continue;  // a step without sort
             const int off = s.bsum[tile.begin / kCellTile];
             // 16-byte words across the block (1 KiB per wavefront load)
@@ -368,30 +370,55 @@ __device__ double profile_u(const SceneDev& s, const double* x, double t) {
     return 0.0;
 }
 
+// The step's end for one scene (one thread): the new particle count and
+// order, or the switch to the step's outputs, and the per-step counters
+// cleared.
+__device__ __forceinline__ void sort_finish(const SceneDev& s, int mode) {
+    SceneDyn* d = s.dyn;
+    if (d->sorted) {
+        if (mode == 0 && (s.flags & FLAG_IO)) {
+            const int room = s.cap - d->n_live;  // n_live is still the step's input count here
+            d->next_fluid_id += d->nspawn < room ? d->nspawn : room;
+            d->next_outlet_id += d->nleave;
+        }
+        d->n_live = s.cell_start[s.ncell];
+        d->outside = d->outside_acc;
+        d->outside_acc = 0;
+        d->parity ^= 1;  // the sort wrote the other order, per-step copy dpar
+        d->rebuild = 1;
+        d->nsorts += 1;
+    } else {
+        d->dpar ^= 1;  // the step's outputs become the current state
+        d->rebuild = 0;
+    }
+    d->novf = 0;
+    d->shepard_ran = 0;
+    d->nspawn = 0;
+    d->nleave = 0;
+    d->disp2_bits = 0;
+    if (mode == 0) d->step += 1;
+}
+
 // Final position = cell start + number of cell members with a smaller
 // entry index (previous order; a step's fluid copies of inlet particles come
 // after every existing particle, in inlet order); gathers every field into
 // the other set.  With the inlet/outlet solver it also applies the step's
 // migration (kinds and group sequence numbers) and the buffer velocities.
-__global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restrict__ scenes,
-                                                         const Tile* __restrict__ tiles, int ntiles, int mode) {
-    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
-        const int t = tile_index(tb, ntiles);
-        const Tile tile = tiles[t];
-        const SceneDev& s = scenes[tile.scene];
-        const SceneDyn dyn = *s.dyn;
+__device__ __forceinline__ void scatter_particle(const SceneDev& s, const SceneDyn& dyn, const Tile& tile,
+                                                 int mode) {
+    {
         if (!dyn.sorted) {
             // a step without sort keeps the particle order: its outputs are
-            // already in the other per-step copy (k_sort_finish switches to
-            // it); only a Shepard step's densities move, into the order's rho
+            // already in the other per-step copy (sort_finish switches to it);
+            // only a Shepard step's densities move, into the order's rho
             const int i = tile.begin + (int)threadIdx.x;
             if (dyn.shepard_ran && i < dyn.n_live) s.set[cur_set(dyn)].rho[i] = s.rhon[i];
-            continue;
+            return;
         }
         const bool io = mode == 0 && (s.flags & FLAG_IO);
         const int ns = io ? spawn_count(s, dyn) : 0;
         const int q = tile.begin + (int)threadIdx.x;
-        if (q >= dyn.n_live + ns) continue;
+        if (q >= dyn.n_live + ns) return;
         const bool spawn = q >= dyn.n_live;
         int src, key, sp_rank = 0;
         if (spawn) {
@@ -402,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
         } else {
             src = q;
             key = s.key[q];
-            if (key == s.ncell) continue;  // deleted
+            if (key == s.ncell) return;  // deleted
         }
         const int b = s.cell_start[key], e = s.cell_start[key + 1];
         int rank = 0;
@@ -504,33 +531,55 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restr
     }
 }
 
-__global__ void k_sort_finish(const SceneDev* __restrict__ scenes, int nscenes, int mode) {
-    const int sc = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sc >= nscenes) return;
-    const SceneDev& s = scenes[sc];
-    SceneDyn* d = s.dyn;
-    if (d->sorted) {
-        if (mode == 0 && (s.flags & FLAG_IO)) {
-            const int room = s.cap - d->n_live;  // n_live is still the step's input count here
-            d->next_fluid_id += d->nspawn < room ? d->nspawn : room;
-            d->next_outlet_id += d->nleave;
-        }
-        d->n_live = s.cell_start[s.ncell];
-        d->outside = d->outside_acc;
-        d->outside_acc = 0;
-        d->parity ^= 1;  // the sort wrote the other order, per-step copy dpar
-        d->rebuild = 1;
-        d->nsorts += 1;
-    } else {
-        d->dpar ^= 1;  // the step's outputs become the current state
-        d->rebuild = 0;
+// The scatter of every particle tile, and the step's end (sort_finish) by
+// the scene's first tile.  The tiles read the step state from the copy
+// taken before this launch (SceneDev::snap), so that end can be written
+// while other tiles still run; the kernels after this one see the new state.
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(const SceneDev* __restrict__ scenes,
+                                                         const Tile* __restrict__ tiles, int ntiles, int mode) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
+        const Tile tile = tiles[t];
+        const SceneDev& s = scenes[tile.scene];
+        const SceneDyn dyn = *s.snap;
+        scatter_particle(s, dyn, tile, mode);
+        if (tile.begin == 0 && threadIdx.x == 0) sort_finish(s, mode);
     }
-    d->novf = 0;
-    d->shepard_ran = 0;
-    d->nspawn = 0;
-    d->nleave = 0;
-    d->disp2_bits = 0;
-    if (mode == 0) d->step += 1;
+}
+
+// Candidate lists of the new order after a sort (every fluid particle within
+// Rv = R (1 + skin) of each fluid particle, and the wall particles within Rv
+// where the cell is near a wall), for every scene but the brick-walk ones,
+// whose continuity pass builds them with the candidates' LDS slots.  A
+// launch of its own: without the pair arithmetic beside it, the build takes
+// fewer registers than the continuity pass and keeps more wavefronts in
+// flight to cover its gathers.  Steps without a sort keep the lists.
+__global__ __launch_bounds__(kBlock) void k_rebuild(const SceneDev* __restrict__ scenes,
+                                                    const Tile* __restrict__ tiles, int ntiles) {
+    for (int tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+        const int t = tile_index(tb, ntiles);
+        const Tile tile = tiles[t];
+        const SceneDev& s = scenes[tile.scene];
+        const SceneDyn dyn = *s.dyn;
+        if (!dyn.rebuild || (s.flags & FLAG_BRICK)) continue;
+        const int i = tile.begin + (int)threadIdx.x;
+        if (i >= dyn.n_live) continue;
+        const FluidSet& S = s.set[cur_set(dyn)];
+        if (S.kind && S.kind[i] != KIND_FLUID) {
+            // inlet/outlet buffers walk no pairs
+            s.nvnb[i] = 0;
+            if (s.nw > 0) s.nnbw[i] = 0;
+            continue;
+        }
+        double xi[3];
+        if (s.dim == 3) {
+            load_x<3>(S.x, i, xi);
+            rebuild_lists<3>(s, S, xi, i);
+        } else {
+            load_x<2>(S.x, i, xi);
+            rebuild_lists<2>(s, S, xi, i);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -590,7 +639,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const SceneDev* __restrict__ s
 // rule over the wall candidates; particles whose lists overflowed take the
 // cell scan of the SCAN instances.  Directed pairs (id_i, id_j): both orders
 // of a fluid-fluid pair appear when both walks decide it.  Lists that the
-// next continuity pass would rebuild are rebuilt here the same way first.
+// next continuity pass would rebuild (brick walks, after a sort) are rebuilt
+// here the same way first.
 template <int D, typename T>
 __global__ __launch_bounds__(kBlock) void k_walk_pairs(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles, int ntiles, int pairset,
@@ -606,7 +656,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_pairs(const SceneDev* __restric
         const FluidSet& S = s.set[cur_set(dyn)];
         double xi[3];
         load_x<D>(S.x, i, xi);
-        if (dyn.rebuild || !s.verlet) rebuild_lists<D>(s, S, xi, i);
+        // (brick-walk scenes build their lists in the continuity pass)
+        if ((dyn.rebuild || !s.verlet) && (s.flags & FLAG_BRICK)) rebuild_lists<D>(s, S, xi, i);
         int64_t w = offs ? offs[i] : 0;
         int cnt = 0;
         const int idi = S.id[i];
@@ -723,7 +774,7 @@ hipError_t launch_sort(const SortPlan& p, int mode, hipStream_t st) {
     hipLaunchKernelGGL(k_sort_offsets_place, dim3(grid_for(p.nctiles + p.nptiles)), blk, 0, st, p.scenes, p.ctiles,
                        p.nctiles, p.ptiles, p.nptiles, mode);
     hipLaunchKernelGGL(k_sort_scatter, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles, mode);
-    hipLaunchKernelGGL(k_sort_finish, dim3((p.nscenes + 63) / 64), dim3(64), 0, st, p.scenes, p.nscenes, mode);
+    hipLaunchKernelGGL(k_rebuild, dim3(grid_for(p.nptiles)), blk, 0, st, p.scenes, p.ptiles, p.nptiles);
     return hipGetLastError();
 }
 

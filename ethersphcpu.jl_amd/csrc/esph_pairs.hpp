@@ -11,9 +11,10 @@ namespace esph {
 //   drho_i = sum_j m_j (v_i - v_j) . gradW_ij,  gradW_ij = W'(r) r_ij / r
 // then updateDensity! (SelfAction/UpdateDensity.jl:8-12) and updatePressure!
 // (SelfAction/UpdatePressure.jl:8-16).  Writes rho, p, c and the record's
-// (rho, p/rho^2).  Right after a sort this pass also builds the candidate
-// lists (every particle within Rv = R (1 + skin)), which all pair passes walk
-// until the next sort, applying the exact rule d^2 <= R^2 to each candidate.
+// (rho, p/rho^2).  It walks the candidate lists (every particle within
+// Rv = R (1 + skin)) that each sort's last launch builds (k_rebuild; brick
+// walks build them here, right after a sort), as all pair passes do until
+// the next sort, applying the exact rule d^2 <= R^2 to each candidate.
 // Candidates beyond R are weighted by zero rather than branched around: the
 // sums equal those over the neighbours alone, and the wavefront stays
 // convergent.
@@ -45,14 +46,12 @@ __device__ __forceinline__ void continuity_particle(const SceneDev& s, const Sce
     load_A<D, T>(S.A, i, vi, m_i, gap_i);
     int cnt = 0;
     if (!SCAN) {
-        if ((dyn.rebuild || !s.verlet) && !dry) {
-            // right after a sort: candidate lists within Rv from the cells
-            if constexpr (BRICK)
-                cnt = rebuild_lists<D>(s, S, xi, i, bm);
-            else
-                cnt = rebuild_lists<D>(s, S, xi, i);
+        if constexpr (BRICK) {
+            // right after a sort: candidate lists within Rv from the cells,
+            // with the candidates' LDS slots
+            cnt = (dyn.rebuild || !s.verlet) && !dry ? rebuild_lists<D>(s, S, xi, i, bm) : s.nvnb[i];
         } else {
-            cnt = s.nvnb[i];
+            cnt = s.nvnb[i];  // built after each sort (k_rebuild)
         }
         if (dry && overflowed(s, i)) return;
         if (overflowed(s, i)) {
