@@ -135,17 +135,14 @@ __device__ __forceinline__ void for_each_row(const SceneDev& s, const int32_t* _
             const float ry = dmin(dy, 1);
             const float r2 = ry * ry + rz * rz;
             if (r2 > lim) continue;
-            int xl = -mx, xh = mx;
-            while (xl < 0) {
-                const float d = dmin(xl, 0);
-                if (d * d + r2 <= lim) break;
-                ++xl;
-            }
-            while (xh > 0) {
-                const float d = dmin(xh, 0);
-                if (d * d + r2 <= lim) break;
-                --xh;
-            }
+            // the cells along x within the distance left, ext: at offset
+            // o > 0 the slab is (o - 1) cs + (cs - p0) - slack away, at o < 0
+            // (-o - 1) cs + p0 - slack (one square root; the float roundings
+            // are far inside the slack, so no cell holding a candidate is cut)
+            const float ext = __builtin_sqrtf(lim - r2), csx = (float)s.cs[0], icx = (float)s.inv_cs[0];
+            const float sl = 1e-5f * csx;
+            const int nl = (int)floorf((ext - p[0] + sl) * icx) + 1, nh = (int)floorf((ext + p[0] + sl) * icx);
+            const int xl = -(nl < 0 ? 0 : (nl > mx ? mx : nl)), xh = nh < 0 ? 0 : (nh > mx ? mx : nh);
             const int cx0 = c[0] + xl > 0 ? c[0] + xl : 0;
             const int cx1 = c[0] + xh < s.nc[0] - 1 ? c[0] + xh : s.nc[0] - 1;
             if (cx0 > cx1) continue;
