@@ -557,7 +557,9 @@ __device__ __forceinline__ bool qgeom_f(const SceneDev& s, const double* __restr
     if (D == 2) rv[2] = 0.0f;
     d2 = D == 3 ? rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2] : rv[0] * rv[0] + rv[1] * rv[1];
     bool in = d2 <= s.R2q;
-    if (qi[0] == kQInvalid || qj[0] == kQInvalid || fabsf(d2 - s.R2q) <= s.qbandq) {
+    // (bitwise: the common path computes everything and takes one branch,
+    // around the rare fallback)
+    if ((qi[0] == kQInvalid) | (qj[0] == kQInvalid) | (fabsf(d2 - s.R2q) <= s.qbandq)) {
         double xj[3], rvd[3];
         load_x<D>(xall, getj(), xj);
         const double d2d = dist2_exact<D>(xi, xj, rvd);
