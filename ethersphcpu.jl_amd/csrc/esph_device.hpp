@@ -51,8 +51,9 @@ enum : int {
     // candidates are addressed by their LDS slots (SceneDev::vslot)
     FLAG_BRICK = 32,
     // the force pass walks each particle's wall candidates in its own lane
-    // (and brick walks always do); without it the candidates of a wavefront's
-    // particles are shared out over all its lanes (wall_walk_shared)
+    // (as brick walks and 2D scenes always do); without it the 3D candidates
+    // of a wavefront's particles are shared out over all its lanes
+    // (wall_walk_shared)
     FLAG_WALL_LANES = 64,
 };
 

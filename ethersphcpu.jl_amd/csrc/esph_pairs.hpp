@@ -797,9 +797,13 @@ __attribute__((amdgpu_waves_per_eu(EOSW && !SCAN ? kEosWalkWaves
                                    : std::is_same<T, float>::value && !THERMAL && !SCAN ? 4 : 1))) void k_forces(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, int ntiles) {
     constexpr int SUB = SCAN ? 1 : kBlock / BS;  // blocks per tile
-    // the wavefront form of the body (list walks without heat): LDS for the
-    // shared wall walk, 64 entries per wavefront
-    constexpr bool WAVE = !SCAN && !THERMAL;
+    // the wavefront form of the body (3D list walks without heat): LDS for
+    // the shared wall walk, 64 entries per wavefront.  2D scenes walk their
+    // walls lane by lane: the wavefront form takes the 2D float pass from 94
+    // registers (five waves per SIMD) to 106 (four), and on the 256-scene 2D
+    // ensemble the shared walk gained nothing (1.678 against 1.685 ms per
+    // batched step)
+    constexpr bool WAVE = !SCAN && !THERMAL && D == 3;
     __shared__ WallOwner<T> wslot[WAVE ? BS : 1];
     __shared__ int wown[WAVE ? BS * kWallWindows : 1];
     const int nv = ntiles * SUB;

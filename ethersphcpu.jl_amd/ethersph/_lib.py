@@ -199,9 +199,10 @@ class Context:
         self.check(self._L.esph_ctx_set_brick_walk(self.handle, 1 if enable else 0))
 
     def set_wall_lanes(self, enable: bool = True) -> None:
-        """Force-pass wall walks in each particle's own lane (True) or shared
-        out over the wavefront's lanes (default); applies to scenes uploaded
-        afterwards.  The two differ in the rounding of the wall sums only."""
+        """3D force-pass wall walks in each particle's own lane (True) or
+        shared out over the wavefront's lanes (default; 2D scenes always walk
+        per lane); applies to scenes uploaded afterwards.  The two differ in
+        the rounding of the wall sums only."""
         self.check(self._L.esph_ctx_set_wall_lanes(self.handle, 1 if enable else 0))
 
     def tuned_workgroup(self, dim: int, kernel: int, precision: int, thermal_mix: int = 1):

@@ -262,15 +262,16 @@ int esph_ctx_set_workgroup(esph_ctx* ctx, int32_t threads);
    0.27 against 0.38 ms per force pass).  Results are identical either way.
    Applies to scenes whose fluid is first uploaded afterwards. */
 int esph_ctx_set_brick_walk(esph_ctx* ctx, int32_t enable);
-/* Wall walks of the force pass (no counterpart in the reference: a choice
-   between two forms of this library's wall loop).  enable 0 (default): the
-   wall candidates of a wavefront's 64 particles are shared out over all its
-   lanes and each particle's wall terms summed by a segmented sum over the
-   lanes (on the 1M-particle 3D dam break a tenth of the lanes of the
-   per-particle form have wall candidates to walk); 1: each particle walks
-   its own wall candidates, adding the terms in list order, as brick walks
-   always do.  The two differ in the rounding of the wall sums only.
-   Applies to scenes whose fluid is first uploaded afterwards. */
+/* Wall walks of the force pass in 3D (no counterpart in the reference: a
+   choice between two forms of this library's wall loop).  enable 0
+   (default): the wall candidates of a wavefront's 64 particles are shared
+   out over all its lanes and each particle's wall terms summed by a
+   segmented sum over the lanes (on the 1M-particle 3D dam break a tenth of
+   the lanes of the per-particle form have wall candidates to walk); 1: each
+   particle walks its own wall candidates, adding the terms in list order,
+   as brick walks and 2D scenes always do.  The two differ in the rounding
+   of the wall sums only.  Applies to scenes whose fluid is first uploaded
+   afterwards. */
 int esph_ctx_set_wall_lanes(esph_ctx* ctx, int32_t enable);
 /* The depth chosen for a class (thermal_mix: 1 plain scenes, 2 thermal, 3
    both in one batch bucket) and, if ms3 is not NULL, the force-pass times
