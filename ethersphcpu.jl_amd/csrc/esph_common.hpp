@@ -593,7 +593,12 @@ struct QScale {
 // The list walk of for_pairs_d with the candidates' geometry from their
 // quantized positions (one 16-byte or 8-byte load instead of the binary64
 // position): body(j, in, d2, rv, record, w) with in = the FP64 rule's
-// decision and w the candidate's FluidSet::xq w bits.
+// decision and w the candidate's FluidSet::xq w bits.  The slots of a
+// group past the particle's count are walked too, with in = false: their
+// list entries are earlier candidates of the same list (or particle 0, the
+// buffer's initial entry), so every value is finite and bodies that weight
+// their terms by in (or gate on it) add nothing for them, without a branch
+// per candidate.
 // Gather groups as in for_pairs_d (SPLIT: the positions, then the records).
 template <int D, int DEPTH, bool SPLIT = true, typename Load, typename Body>
 __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S, const double* xi, int i,
@@ -631,11 +636,10 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                if (u < m) {
-                    float rv[3], d2;
-                    const bool in = qgeom<D>(s, S.x, xi, qi, qj[u], js[u], rv, d2);
-                    body(js[u], in, d2, rv, rec[u], qj[u][3]);
-                }
+                // slots past the count: weighted out, not branched around
+                float rv[3], d2;
+                const bool in = qgeom<D>(s, S.x, xi, qi, qj[u], js[u], rv, d2) & (u < m);
+                body(js[u], in, d2, rv, rec[u], qj[u][3]);
             }
             c0 = n0;
             c1 = n1;
@@ -663,11 +667,10 @@ __device__ __forceinline__ void for_pairs_q(const SceneDev& s, const FluidSet& S
                 }
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) {
-                    if (g + u < m) {
-                        float rv[3], d2;
-                        const bool in = qgeom<D>(s, S.x, xi, qi, qj[u], js[g + u], rv, d2);
-                        body(js[g + u], in, d2, rv, rec[u], qj[u][3]);
-                    }
+                    // slots past the count: weighted out, not branched around
+                    float rv[3], d2;
+                    const bool in = qgeom<D>(s, S.x, xi, qi, qj[u], js[g + u], rv, d2) & (g + u < m);
+                    body(js[g + u], in, d2, rv, rec[u], qj[u][3]);
                 }
             }
         }
