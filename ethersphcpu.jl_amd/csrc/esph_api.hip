@@ -554,7 +554,7 @@ static int allocate_scene(esph_scene* s, const esph_fluid_arrays* f) {
     {
         // candidate lists reused across steps (not for the inlet/outlet
         // solver, whose migrations reorder the particles every step)
-        const double skin = d.verlet_skin == 0.0 ? kDefaultSkin : d.verlet_skin;
+        const double skin = d.verlet_skin == 0.0 ? default_skin(D) : d.verlet_skin;
         hd.verlet = (skin > 0.0 && !io) ? 1 : 0;
         const double sk = hd.verlet ? skin : 0.0;
         hd.Rv = d.influence_radius * (1.0 + sk);

@@ -62,9 +62,13 @@ enum : int8_t { KIND_FLUID = 0, KIND_INLET = 1, KIND_OUTLET = 2 };
 
 // Candidate-list skin (fraction of R) when the descriptor leaves it at 0.
 // Every pair pass walks the candidates, so their excess (1 + skin)^D - 1
-// costs on every step and a rebuild on every skin R / 2 of motion: swept on
-// the 1M-particle 3D dam break, 0.03 is best (0.02 and 0.04 within 1%).
-constexpr double kDefaultSkin = 0.03;
+// costs on every step and a sort with a list rebuild on every skin R / 2 of
+// motion.  Swept over 1,000 steps (FP32_BULK, ms per step, sorts): the 1M-
+// particle 3D dam break 0.02: 0.427 (63), 0.03: 0.413 (41), 0.04: 0.406
+// (31), 0.05: 0.403 (25), 0.06: 0.3995 (20), 0.07: 0.400 (17), 0.08: 0.402
+// (15); the 256-scene 2D ensemble, ms per batched step, 0.03: 1.646, 0.045:
+// 1.613, 0.06: 1.665.
+__host__ __device__ inline double default_skin(int dim) { return dim == 3 ? 0.06 : 0.045; }
 
 // Per-scene state that changes every step and lives in device memory, so a
 // captured step graph can be replayed without the host.

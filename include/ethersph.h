@@ -190,7 +190,8 @@ typedef struct esph_scene_desc {
     /* candidate-list skin as a fraction of R: the lists hold every particle
        within R (1 + skin) and are reused, filtered by the exact rule, until
        some particle has moved skin R / 2 (or a check or migration step
-       needs a sort).  0: the library default (0.03); negative: the lists are
+       needs a sort).  0: the library default (0.06 in 3D, 0.045 in 2D);
+       negative: the lists are
        rebuilt every step.  The inlet/outlet solver rebuilds every step.
        Results follow the same neighbour rule either way. */
     double verlet_skin;

@@ -377,7 +377,7 @@ def test_overflow_scan_path_matches_oracle(ctx):
 
 @pytest.mark.parametrize("skin", [-1.0, 0.0, 0.3])
 def test_candidate_list_reuse_keeps_parity(ctx, skin):
-    """Candidate lists reused across steps (skin 0.03 by default, 0.3) or
+    """Candidate lists reused across steps (skin 0.045 by default in 2D, 0.3) or
     rebuilt every step (negative skin): the same pair sets and fields as the
     oracle; with reuse, fewer sorts than steps."""
     sc = scenarios.collapse_dry_2d(dx=0.04)
