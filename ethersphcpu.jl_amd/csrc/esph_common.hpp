@@ -395,8 +395,10 @@ struct has_late<L, std::void_t<decltype(&L::late)>> : std::true_type {};
 // the current one is used.  Every slot of a written chunk holds a valid
 // particle index (slots past the count keep earlier entries or zero), and a
 // group past the last chunk reuses an earlier one, so a group may load past
-// the count; body runs only for the counted entries, in list order, so the
-// result depends neither on DEPTH nor on WIDE.
+// the count; body runs for the counted entries in list order and, without a
+// branch, for the slots past the count with d^2 = 4 R^2, which every body's
+// exact rule rejects (its terms add zero), so the result depends neither on
+// DEPTH nor on WIDE.
 template <int D, bool SCAN, int DEPTH, bool SPLIT = true, bool WIDE = false, typename Load, typename Body>
 __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __restrict__ cstart,
                                             const double* __restrict__ xall, const double* xi, int i, int self,
@@ -416,6 +418,11 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
     if (cnt <= 0) return;
     const size_t cap = (size_t)s.cap;
     const int nq = (cnt + 3) >> 2;
+    // d^2 handed to the body for a group's slots past the count: their list
+    // entries are earlier candidates of the same list (or index 0, the
+    // buffer's initial entry), so their data are finite, and at 4 R^2 (twice
+    // R, beyond every kernel's support) every body's exact rule rejects them
+    const double beyond = 4.0 * s.R2;
     if constexpr (WIDE && DEPTH == 4) {
         // two chunks (8 candidates) per group, as in for_pairs_q
         int4 c0 = ld_chunk(list + list_at(cap, 0, i));
@@ -438,11 +445,10 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                if (u < m) {
-                    double rv[3];
-                    const double d2 = dist2_exact<D>(xi, xj[u], rv);
-                    body(js[u], d2, rv, rec[u]);
-                }
+                // slots past the count: pushed beyond R, not branched around
+                double rv[3];
+                const double d2 = dist2_exact<D>(xi, xj[u], rv);
+                body(js[u], u < m ? d2 : beyond, rv, rec[u]);
             }
             c0 = n0;
             c1 = n1;
@@ -470,11 +476,10 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
                 }
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) {
-                    if (g + u < m) {
-                        double rv[3];
-                        const double d2 = dist2_exact<D>(xi, xj[u], rv);
-                        body(js[g + u], d2, rv, rec[u]);
-                    }
+                    // slots past the count: pushed beyond R, not branched around
+                    double rv[3];
+                    const double d2 = dist2_exact<D>(xi, xj[u], rv);
+                    body(js[g + u], g + u < m ? d2 : beyond, rv, rec[u]);
                 }
             }
         }
