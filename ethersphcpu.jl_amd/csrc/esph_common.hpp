@@ -395,11 +395,16 @@ struct has_late<L, std::void_t<decltype(&L::late)>> : std::true_type {};
 // the current one is used.  Every slot of a written chunk holds a valid
 // particle index (slots past the count keep earlier entries or zero), and a
 // group past the last chunk reuses an earlier one, so a group may load past
-// the count; body runs for the counted entries in list order and, without a
-// branch, for the slots past the count with d^2 = 4 R^2, which every body's
-// exact rule rejects (its terms add zero), so the result depends neither on
-// DEPTH nor on WIDE.
-template <int D, bool SCAN, int DEPTH, bool SPLIT = true, bool WIDE = false, typename Load, typename Body>
+// the count; body runs for the counted entries in list order (TAILW: also,
+// without a branch, for the slots past the count with d^2 = 4 R^2, which
+// every body's exact rule rejects, its terms adding zero -- for bodies with
+// float arithmetic, where the extra terms cost about what the branch per
+// candidate does: the 2D ensemble 1.603 against 1.613 ms per batched step;
+// in binary64, with its costly divisions and square roots, the FP64 3D dam
+// break took 0.836 against 0.800 ms per step), so the result depends
+// neither on DEPTH nor on WIDE.
+template <int D, bool SCAN, int DEPTH, bool SPLIT = true, bool WIDE = false, bool TAILW = false, typename Load,
+          typename Body>
 __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __restrict__ cstart,
                                             const double* __restrict__ xall, const double* xi, int i, int self,
                                             const int32_t* __restrict__ list, int cnt, Load&& load, Body&& body) {
@@ -445,10 +450,11 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                // slots past the count: pushed beyond R, not branched around
-                double rv[3];
-                const double d2 = dist2_exact<D>(xi, xj[u], rv);
-                body(js[u], u < m ? d2 : beyond, rv, rec[u]);
+                if (TAILW || u < m) {
+                    double rv[3];
+                    const double d2 = dist2_exact<D>(xi, xj[u], rv);
+                    body(js[u], !TAILW || u < m ? d2 : beyond, rv, rec[u]);
+                }
             }
             c0 = n0;
             c1 = n1;
@@ -476,10 +482,11 @@ __device__ __forceinline__ void for_pairs_d(const SceneDev& s, const int32_t* __
                 }
 #pragma unroll
                 for (int u = 0; u < DEPTH; ++u) {
-                    // slots past the count: pushed beyond R, not branched around
-                    double rv[3];
-                    const double d2 = dist2_exact<D>(xi, xj[u], rv);
-                    body(js[g + u], g + u < m ? d2 : beyond, rv, rec[u]);
+                    if (TAILW || g + u < m) {
+                        double rv[3];
+                        const double d2 = dist2_exact<D>(xi, xj[u], rv);
+                        body(js[g + u], !TAILW || g + u < m ? d2 : beyond, rv, rec[u]);
+                    }
                 }
             }
         }

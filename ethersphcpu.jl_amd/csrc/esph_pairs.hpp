@@ -109,7 +109,7 @@ __device__ __forceinline__ void continuity_particle(const SceneDev& s, const Sce
             for_pairs_q<D, DEPTH>(s, S, xi, i, s.vnbr, cnt, load, qbody);
         }
     } else {
-        for_pairs_d<D, SCAN, DEPTH, true, D == 3>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
+        for_pairs_d<D, SCAN, DEPTH, true, D == 3, sizeof(T) == 4>(s, s.cell_start, S.x, xi, i, i, s.vnbr, cnt, load,
                                     [&](int, double d2, const double* rv, const ARec<T>& a) {
             T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
             pair(inv_h, d2 <= R2 ? T(1) : T(0), (T)d2, rvt, a);
@@ -637,7 +637,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
             ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::false_type{}, gq);
         });
     } else {
-        for_pairs_d<D, SCAN, DEPTH, false>(s, s.cell_start, S.x, xi, i, i, s.vnbr, nv,
+        for_pairs_d<D, SCAN, DEPTH, false, false, sizeof(T) == 4>(s, s.cell_start, S.x, xi, i, i, s.vnbr, nv,
                                            RecLoaderAB<D, T, THERMAL>{S},
                                            [&](int, double d2, const double* rv, const Rec<T>& o) {
             const T rvt[3] = {(T)rv[0], (T)rv[1], D == 3 ? (T)rv[2] : T(0)};
@@ -694,7 +694,8 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
             // (brick walks: at most two wall gathers in flight, which keeps the
             // four-deep fluid walk within the registers of four waves per SIMD)
             constexpr int WDEPTH = BRICK && DEPTH > 2 ? 2 : DEPTH;
-            for_pairs_d<D, SCAN, WDEPTH, SPLIT>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr, WAVE ? cw : s.nnbw[i], wload,
+            for_pairs_d<D, SCAN, WDEPTH, SPLIT, false, sizeof(T) == 4>(s, s.wcell_start, s.wx, xi, i, -1, s.wnbr,
+                                                                    WAVE ? cw : s.nnbw[i], wload,
                                                 [&](int j, double d2, const double* rv, const WRec& w) {
                 // candidates within Rv: the exact rule selects the terms (no branch)
                 const bool in = d2 <= wc.R2;
