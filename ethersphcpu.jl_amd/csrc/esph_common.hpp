@@ -697,17 +697,6 @@ struct ARec {
     T m, gap;
 };
 
-// Inclusive scan over a wavefront (64 lanes).
-__device__ __forceinline__ int wave_inclusive_scan(int x) {
-    const int lane = (int)(threadIdx.x & 63);
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    return x;
-}
-
 // DPP moves (GFX9 controls): by n lanes toward higher lanes inside each
 // 16-lane row (kDppRowShr + n), lane 15 of each row to the next row, lane 31
 // to rows 2 and 3; lanes outside the row mask, or whose source lies outside
@@ -753,6 +742,10 @@ __device__ __forceinline__ T wave_segmented_sum(T v, int head) {
     v += lane >= 32 && head < 32 ? y : T(0);
     return v;
 }
+
+// Inclusive scan over a wavefront (64 lanes, every lane active): one
+// segment, in DPP moves (LDS-free; a shuffle per step waits on LDS).
+__device__ __forceinline__ int wave_inclusive_scan(int x) { return wave_segmented_sum(x, 0); }
 
 // Orders LDS accesses of one wavefront's lanes to each other (its LDS
 // instructions complete in program order; this keeps the compiler's order).

@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan1(const SceneDev* __restric
             const int sum = w.x + w.y + w.z + w.w;
             const int incl = wave_inclusive_scan(sum);
             ex[k] = carry + incl - sum;
-            carry += __shfl(incl, 63, 64);
+            carry += __builtin_amdgcn_readlane(incl, 63);
         }
         __shared__ int wtot[kBlock / 64];
         if (lane == 0) wtot[wave] = carry;
