@@ -530,8 +530,11 @@ __device__ __forceinline__ void load_q(const void* xq, int j, int32_t* q) {
 // (rho / rho0)^gamma - 1 = e (1 + u + ... + u^(gamma-1)), e = d / rho0, u = 1 + e;
 // p = b e (...) + p0.  Also returns rho = rho0 + d.
 // LIN: gamma is 1 and p0 is 0 (the linear equation of state of the
-// examples), so p = (b / rho0) d with no loop.
-template <bool LIN = false>
+// examples), so p = (b / rho0) d with no loop.  GAMMA > 0: gamma is GAMMA,
+// the sum unrolled (the same multiply-adds in the same order as the loop
+// over s.gamma, without a uniform branch per term).
+constexpr int kTaitGamma = 7;  // the exponent of the Tait equation of state most scenes use
+template <bool LIN = false, int GAMMA = 0>
 __device__ __forceinline__ float eos_pr2(const SceneDev& s, float d, float& rho) {
     float p;
     if (LIN) {
@@ -540,7 +543,12 @@ __device__ __forceinline__ float eos_pr2(const SceneDev& s, float d, float& rho)
         const float e = d * s.inv_rho0f;
         const float u = 1.0f + e;
         float g = 1.0f;
-        for (int k = 1; k < s.gamma; ++k) g = g * u + 1.0f;
+        if constexpr (GAMMA > 0) {
+#pragma unroll
+            for (int k = 1; k < GAMMA; ++k) g = g * u + 1.0f;
+        } else {
+            for (int k = 1; k < s.gamma; ++k) g = g * u + 1.0f;
+        }
         p = s.bf * (e * g) + s.p0f;
     }
     rho = s.rho0f + d;

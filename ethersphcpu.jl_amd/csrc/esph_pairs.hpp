@@ -591,6 +591,7 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
             // alike, for this particle
             float rho_e;
             const bool lin = s.gamma == 1 && s.p0f == 0.0f;
+            const bool tait = !lin && s.gamma == kTaitGamma;
             const float d_i = reinterpret_cast<const float*>(S.xq)[4 * (size_t)i + 3];
             pr2_i = lin ? eos_pr2<true>(s, d_i, rho_e) : eos_pr2<false>(s, d_i, rho_e);
             auto loadA = [&](int j) {
@@ -598,17 +599,18 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 load_A<D, T>(S.A, j, a.v, a.m, a.gap);
                 return a;
             };
-            // the walk in two forms, chosen once per particle: the linear
-            // equation of state (gamma 1, p0 0: no loop per candidate) or any
-            auto walk = [&](auto lin) {
-                constexpr bool LIN = decltype(lin)::value;
+            // the walk in three forms, chosen once per particle: the linear
+            // equation of state (gamma 1, p0 0: no sum per candidate), gamma
+            // 7 (the sum unrolled) or any gamma (a loop per candidate)
+            auto walk = [&](auto form) {
+                constexpr int FORM = decltype(form)::value;  // 1 linear, kTaitGamma, 0 any
                 auto body = [&](int, bool in, float d2, const float* rv, const ARec<T>& a, int32_t w) {
                     Rec<T> o;
                     o.v[0] = a.v[0];
                     o.v[1] = a.v[1];
                     o.v[2] = a.v[2];
                     o.m = a.m;
-                    o.pr2 = eos_pr2<LIN>(s, __int_as_float(w), o.rho);
+                    o.pr2 = eos_pr2<FORM == 1, FORM == kTaitGamma ? kTaitGamma : 0>(s, __int_as_float(w), o.rho);
                     ff(in ? T(1) : T(0), d2 == 0.0f, d2, rv, o, std::true_type{}, gq);
                 };
                 if constexpr (BRICK) {
@@ -624,9 +626,11 @@ __device__ __forceinline__ double forces_body(const SceneDev& s, const FluidSet&
                 }
             };
             if (lin)
-                walk(std::true_type{});
+                walk(std::integral_constant<int, 1>{});
+            else if (tait)
+                walk(std::integral_constant<int, kTaitGamma>{});
             else
-                walk(std::false_type{});
+                walk(std::integral_constant<int, 0>{});
         }
     } else if constexpr (sizeof(T) == 4 && !SCAN && D == 3) {
         // FP32_BULK in 3D: geometry from the quantized positions (FP64 rule
