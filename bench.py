@@ -128,12 +128,14 @@ def ensemble_leg(steps, alone_steps, n_scenes, full_steps=5000):
             s.sync()
         t_plain = (time.perf_counter() - t0) / alone_steps
         # with the scene's own captured step graphs, one scene at a time: each
-        # scene captures, warms up, runs its timed steps and releases its
-        # graphs before the next (at most two live graph executables)
+        # scene captures its graphs in an untimed run as long as the timed one
+        # (single steps, with and without the Shepard pass, and runs of 2-16
+        # steps), runs its timed steps and releases its graphs before the next
+        # (at most six live graph executables)
         t_alone = 0.0
         for s in scenes:
             s.enable_graph(True)
-            s.step(2)
+            s.step(alone_steps)
             s.sync()
             t0 = time.perf_counter()
             s.step(alone_steps)
@@ -153,7 +155,7 @@ def ensemble_leg(steps, alone_steps, n_scenes, full_steps=5000):
             "one_at_a_time_graph_scenes_per_s": alone, "speedup": batched / plain,
             "speedup_vs_graphed_scenes": batched / alone,
             "batched_particle_steps_per_s": npart / t_batch,
-            "sample": f"{steps} batched steps (one graph per step) and {alone_steps} steps per scene alone "
+            "sample": f"{steps} batched steps (step graphs, up to 16 steps in one) and {alone_steps} steps per scene alone "
                       f"(plain launches, then the scene's own graph), FP32_BULK; scenes/s extrapolated to "
                       f"{full_steps}-step scenes"}
 

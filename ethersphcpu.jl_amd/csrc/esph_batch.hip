@@ -267,6 +267,12 @@ void StepPlan::drop_graphs() {
         exec[q] = nullptr;
         graph[q] = nullptr;
     }
+    for (int k = 0; k < kChunkLevels; ++k) {
+        if (cexec[k]) (void)hipGraphExecDestroy(cexec[k]);
+        if (cgraph[k]) (void)hipGraphDestroy(cgraph[k]);
+        cexec[k] = nullptr;
+        cgraph[k] = nullptr;
+    }
 }
 
 int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
@@ -302,21 +308,46 @@ int StepPlan::step(esph_ctx* ctx, int64_t nsteps) {
     // changed) is recaptured
     const std::vector<int> depths = launch_config();
     if (depths != captured_depth) drop_graphs();
-    for (int64_t t = t0; t < t0 + nsteps; ++t) {
+    // captures `steps` steps (with the Shepard pass when shepard) into g / x
+    auto capture = [&](int steps, bool shepard, hipGraph_t& g, hipGraphExec_t& x) -> int {
+        captured_depth = depths;
+        HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < steps && e == hipSuccess; ++k) e = launch_step(st, shepard);
+        hipError_t e2 = hipStreamEndCapture(st, &g);
+        if (e != hipSuccess) return hip_fail(ctx, "graph capture", e);
+        if (e2 != hipSuccess) return hip_fail(ctx, "graph capture end", e2);
+        HIPCHK(ctx, hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+        return ESPH_OK;
+    };
+    const int64_t end = t0 + nsteps;
+    for (int64_t t = t0; t < end;) {
         const int q = shepard_due(t) ? 1 : 0;
+        if (use_graph && q == 0) {
+            // the longest chunk (2, 4, 8 or 16 steps) of the run of steps
+            // without a Shepard pass that starts here
+            int run = 1;
+            while (run < (2 << (kChunkLevels - 1)) && t + run < end && !shepard_due(t + run)) ++run;
+            int k = kChunkLevels - 1;
+            while (k >= 0 && (2 << k) > run) --k;
+            if (k >= 0) {
+                if (!cexec[k]) {
+                    const int rc = capture(2 << k, false, cgraph[k], cexec[k]);
+                    if (rc != ESPH_OK) return rc;
+                }
+                const hipError_t e = hipGraphLaunch(cexec[k], st);
+                if (e != hipSuccess) return hip_fail(ctx, "batch step", e);
+                t += 2 << k;
+                continue;
+            }
+        }
         if (use_graph && !exec[q]) {
-            captured_depth = depths;
-            hipGraph_t g = nullptr;
-            HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            hipError_t e = launch_step(st, q == 1);
-            hipError_t e2 = hipStreamEndCapture(st, &g);
-            if (e != hipSuccess) return hip_fail(ctx, "graph capture", e);
-            if (e2 != hipSuccess) return hip_fail(ctx, "graph capture end", e2);
-            graph[q] = g;
-            HIPCHK(ctx, hipGraphInstantiate(&exec[q], graph[q], nullptr, nullptr, 0));
+            const int rc = capture(1, q == 1, graph[q], exec[q]);
+            if (rc != ESPH_OK) return rc;
         }
         hipError_t e = use_graph ? hipGraphLaunch(exec[q], st) : launch_step(st, q == 1);
         if (e != hipSuccess) return hip_fail(ctx, "batch step", e);
+        ++t;
     }
     return ESPH_OK;
 }

@@ -92,6 +92,12 @@ struct StepPlan {
     bool use_graph = false;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    // runs of steps without a Shepard pass, 2 << k steps in one graph: the
+    // launch of a graph costs a gap of several microseconds before its first
+    // kernel, the kernels inside one follow each other without it
+    static constexpr int kChunkLevels = 4;
+    hipGraph_t cgraph[kChunkLevels] = {};
+    hipGraphExec_t cexec[kChunkLevels] = {};
     std::vector<int> captured_depth;  // bucket depths and workgroup sizes the graphs were captured with
     std::vector<int> launch_config() const;  // those of the buckets now
     int build(esph_ctx* ctx, const std::vector<esph_scene*>& members);

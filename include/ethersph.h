@@ -394,9 +394,10 @@ int esph_scene_stats(esph_scene* scene, int64_t* out8);
    neighbour region exceeded the LDS staging (those walked their candidates
    from device memory; results are the same). */
 int esph_scene_brick_stats(esph_scene* scene, int64_t* out6);
-/* Graph mode: the first step captures one step as a HIP graph; later steps
-   replay it (the Shepard pass is then launched every step and skips itself
-   on the device when not due).  Phase timing is not recorded in graph mode. */
+/* Graph mode: steps are captured as HIP graphs on first use and replayed --
+   one step with the Shepard pass, one without, and runs of 2, 4, 8 or 16
+   steps without it (the host picks from the step count; results are the same
+   as with plain launches).  Phase timing is not recorded in graph mode. */
 int esph_scene_enable_graph(esph_scene* scene, int32_t enable);
 
 /* ---- batches (ensembles) ---------------------------------------------------

@@ -80,6 +80,8 @@ def test_batch_without_graph_and_members_stepped_alone(ctx):
 
 
 def test_scene_graph_equals_plain(ctx):
+    """95 steps replayed from step graphs (single steps with and without the
+    Shepard pass, and runs of 2-16 steps in one graph) against plain launches."""
     sc = scenarios.collapse_dry_2d(dx=0.03)
     a = sc.make_scene(_lib.FP32_BULK, ctx)
     g = sc.make_scene(_lib.FP32_BULK, ctx)
