@@ -97,3 +97,29 @@ def test_batch_tunes_each_class_once():
     for s in scenes:
         s.close()
     ctx.close()
+
+
+def test_graphs_recaptured_when_the_launch_settings_change():
+    """A scene in graph mode whose context changes its workgroup size and
+    gather depth between calls (its plan then drops every step graph, single
+    steps and runs of 2-16 steps, and captures them again with the new
+    settings): the fields equal plain launches with fixed settings bit for
+    bit."""
+    sc = scenarios.collapse_dry_2d(dx=0.03)
+    ref, _ = run(sc, _lib.FP32_BULK, 70, False, 2)
+    ctx = _lib.Context(0)
+    ctx.set_tuning(False, 4)
+    ctx.set_workgroup(64)
+    s = sc.make_scene(_lib.FP32_BULK, ctx)
+    s.enable_graph(True)
+    s.step(35)
+    ctx.set_tuning(False, 1)
+    ctx.set_workgroup(128)
+    s.step(35)
+    s.sync()
+    say("graphs recaptured after a settings change, steps", s.step_count)
+    assert s.step_count == 70
+    for f in FIELDS:
+        assert np.array_equal(s.get(f), ref[f]), ("recaptured graph", f)
+    s.close()
+    ctx.close()
